@@ -1,0 +1,54 @@
+/*
+ * binnedBLAS.h -- binned BLAS over double vectors and matrices, as exported by
+ * libreproblas_hip.so.
+ *
+ * Drop-in for the double-precision part of the reference's
+ * include/binnedBLAS.h (ReproBLAS 2.1.0). The dbd* calls accumulate into a
+ * caller-owned binned value (or vector of them), as the reference's do.
+ *
+ * Operands may live in device memory (hipMalloc / managed) -- then the work
+ * runs in place on the GPU -- or in host memory, which is staged to the GPU
+ * and back. The reductions always run on the GPU; the library has no CPU
+ * compute path. Calls run on the stream set with reproblas_hip_set_stream()
+ * and return after it has finished. Errors are reported through
+ * reproblas_hip_last_error().
+ */
+#ifndef BINNEDBLAS_H_
+#define BINNEDBLAS_H_
+#include "binned.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+double binnedBLAS_damax(const int N, const double *X, const int incX);              /* binnedBLAS.h:38 */
+double binnedBLAS_damaxm(const int N, const double *X, const int incX, const double *Y,
+                         const int incY);                                           /* binnedBLAS.h:43 */
+void binnedBLAS_dbdsum(const int fold, const int N, const double *X, const int incX,
+                       double_binned *Y);                                           /* binnedBLAS.h:47 */
+void binnedBLAS_dmdsum(const int fold, const int N, const double *X, const int incX, double *priY,
+                       const int incpriY, double *carY, const int inccarY);         /* binnedBLAS.h:48 */
+void binnedBLAS_dbdasum(const int fold, const int N, const double *X, const int incX,
+                        double_binned *Y);                                          /* binnedBLAS.h:49 */
+void binnedBLAS_dmdasum(const int fold, const int N, const double *X, const int incX, double *priY,
+                        const int incpriY, double *carY, const int inccarY);        /* binnedBLAS.h:50 */
+double binnedBLAS_dbdssq(const int fold, const int N, const double *X, const int incX, const double scaleY,
+                         double_binned *Y);                                         /* binnedBLAS.h:51 */
+double binnedBLAS_dmdssq(const int fold, const int N, const double *X, const int incX, const double scaleY,
+                         double *priY, const int incpriY, double *carY, const int inccarY); /* binnedBLAS.h:52 */
+void binnedBLAS_dbddot(const int fold, const int N, const double *X, const int incX, const double *Y,
+                       const int incY, double_binned *Z);                           /* binnedBLAS.h:53 */
+void binnedBLAS_dmddot(const int fold, const int N, const double *X, const int incX, const double *Y,
+                       const int incY, double *priZ, const int incpriZ, double *carZ,
+                       const int inccarZ);                                          /* binnedBLAS.h:54 */
+void binnedBLAS_dbdgemv(const int fold, const char Order, const char TransA, const int M, const int N,
+                        const double alpha, const double *A, const int lda, const double *X, const int incX,
+                        double_binned *Y, const int incY);                          /* binnedBLAS.h:87 */
+void binnedBLAS_dbdgemm(const int fold, const char Order, const char TransA, const char TransB, const int M,
+                        const int N, const int K, const double alpha, const double *A, const int lda,
+                        const double *B, const int ldb, double_binned *C, const int ldc); /* binnedBLAS.h:92 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,58 @@
+/*
+ * reproBLAS.h -- reproducible double-precision BLAS, as exported by
+ * libreproblas_hip.so.
+ *
+ * Drop-in for the double-precision part of the reference's
+ * include/reproBLAS.h (ReproBLAS 2.1.0): same names, signatures, fold
+ * semantics and results, bit for bit. The reproBLAS_r* calls take an explicit
+ * fold; the others use DIDEFAULTFOLD (3, reference config.h:14).
+ *
+ * Operands may be device pointers (the work runs in place on the GPU) or host
+ * pointers (staged to the GPU and back). Calls run on the stream set with
+ * reproblas_hip_set_stream() and return once their results are ready.
+ * Argument conventions follow the reference: Order 'r'/'R' is row-major and
+ * anything else column-major; TransA/TransB 'n'/'N' is no-transpose and
+ * anything else transpose; negative increments are not pointer-adjusted.
+ */
+#ifndef REPROBLAS_H_
+#define REPROBLAS_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef DIDEFAULTFOLD
+#define DIDEFAULTFOLD 3 /* reference config.h:14 */
+#endif
+
+double reproBLAS_rdsum(const int fold, const int N, const double *X, const int incX);  /* reproBLAS.h:27 */
+double reproBLAS_rdasum(const int fold, const int N, const double *X, const int incX); /* reproBLAS.h:28 */
+double reproBLAS_rdnrm2(const int fold, const int N, const double *X, const int incX); /* reproBLAS.h:29 */
+double reproBLAS_rddot(const int fold, const int N, const double *X, const int incX, const double *Y,
+                       const int incY);                                                 /* reproBLAS.h:30 */
+
+void reproBLAS_rdgemv(const int fold, const char Order, const char TransA, const int M, const int N,
+                      const double alpha, const double *A, const int lda, const double *X, const int incX,
+                      const double beta, double *Y, const int incY); /* reproBLAS.h:49 */
+void reproBLAS_rdgemm(const int fold, const char Order, const char TransA, const char TransB, const int M,
+                      const int N, const int K, const double alpha, const double *A, const int lda,
+                      const double *B, const int ldb, const double beta, double *C,
+                      const int ldc); /* reproBLAS.h:54 */
+
+double reproBLAS_dsum(const int N, const double *X, const int incX);  /* reproBLAS.h:97 */
+double reproBLAS_dasum(const int N, const double *X, const int incX); /* reproBLAS.h:98 */
+double reproBLAS_dnrm2(const int N, const double *X, const int incX); /* reproBLAS.h:99 */
+double reproBLAS_ddot(const int N, const double *X, const int incX, const double *Y,
+                      const int incY); /* reproBLAS.h:100 */
+
+void reproBLAS_dgemv(const char Order, const char TransA, const int M, const int N, const double alpha,
+                     const double *A, const int lda, const double *X, const int incX, const double beta, double *Y,
+                     const int incY); /* reproBLAS.h:119 */
+void reproBLAS_dgemm(const char Order, const char TransA, const char TransB, const int M, const int N, const int K,
+                     const double alpha, const double *A, const int lda, const double *B, const int ldb,
+                     const double beta, double *C, const int ldc); /* reproBLAS.h:124 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

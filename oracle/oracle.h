@@ -1,0 +1,80 @@
+/*
+ * oracle.h -- CPU restatement of ReproBLAS 2.1.0's binned FP64 reduction path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the correctness checker for the
+ * HIP library in reproblas_amd/. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker or as the
+ * timed CPU baseline. The product never links, loads or calls it.
+ *
+ * Every symbol is prefixed orc_ so it can never be confused with (or
+ * interpose on) the product's reproBLAS_/binnedBLAS_/binned_ symbols.
+ *
+ * A binned value of fold F is double[2F]: F primaries, then F carries
+ * (include/binned.h:46 and src/binned/dbdbadd.c:16-18 of the reference).
+ */
+#ifndef REPROBLAS_ORACLE_H_
+#define REPROBLAS_ORACLE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* constants (reference include/binned.h:84,105,123,142,184,206) */
+#define ORC_DBWIDTH 40
+#define ORC_MAXINDEX 51
+#define ORC_MAXFOLD 52
+#define ORC_ENDURANCE 2048
+
+/* primitives (reference src/binned/) */
+const double *orc_dmbins(int X);
+int orc_dindex(double X);
+int orc_dmindex(const double *priX);
+int orc_dmindex0(const double *priX);
+void orc_dmdupdate(int fold, double X, double *priY, int incpriY, double *carY, int inccarY);
+void orc_dmddeposit(int fold, double X, double *priY, int incpriY);
+void orc_dmrenorm(int fold, double *priX, int incpriX, double *carX, int inccarX);
+void orc_dmdmadd(int fold, const double *priX, int incpriX, const double *carX, int inccarX,
+                 double *priY, int incpriY, double *carY, int inccarY);
+void orc_dbdbadd(int fold, const double *X, double *Y);
+void orc_dmdadd(int fold, double X, double *priY, int incpriY, double *carY, int inccarY);
+void orc_dbdadd(int fold, double X, double *Y);
+void orc_dbdconv(int fold, double X, double *Y);
+double orc_ddmconv(int fold, const double *priX, int incpriX, const double *carX, int inccarX);
+double orc_ddbconv(int fold, const double *X);
+double orc_dscale(double X);
+void orc_dmdrescale(int fold, double X, double scaleY, double *priY, int incpriY, double *carY, int inccarY);
+double orc_dbdbaddsq(int fold, double scaleX, const double *X, double scaleY, double *Y);
+double orc_dbbound(int fold, int N, double X, double S);
+
+/* binned BLAS (reference src/binnedBLAS/) */
+double orc_damax(int N, const double *X, int incX);
+double orc_damaxm(int N, const double *X, int incX, const double *Y, int incY);
+void orc_dbdsum(int fold, int N, const double *X, int incX, double *Y);
+void orc_dbdasum(int fold, int N, const double *X, int incX, double *Y);
+void orc_dbddot(int fold, int N, const double *X, int incX, const double *Y, int incY, double *Z);
+double orc_dbdssq(int fold, int N, const double *X, int incX, double scaleY, double *Y);
+
+/* user API (reference src/reproBLAS/) */
+double orc_rdsum(int fold, int N, const double *X, int incX);
+double orc_rdasum(int fold, int N, const double *X, int incX);
+double orc_rddot(int fold, int N, const double *X, int incX, const double *Y, int incY);
+double orc_rdnrm2(int fold, int N, const double *X, int incX);
+void orc_rdgemv(int fold, char Order, char TransA, int M, int N, double alpha,
+                const double *A, int lda, const double *X, int incX,
+                double beta, double *Y, int incY);
+void orc_rdgemm(int fold, char Order, char TransA, char TransB, int M, int N, int K,
+                double alpha, const double *A, int lda, const double *B, int ldb,
+                double beta, double *C, int ldc);
+
+/* binned-state variants of gemv/gemm: accumulate into caller binned arrays
+ * (reference binnedBLAS_dbdgemv / binnedBLAS_dbdgemm semantics). */
+void orc_dbdgemv(int fold, char Order, char TransA, int M, int N, double alpha,
+                 const double *A, int lda, const double *X, int incX, double *Y, int incY);
+void orc_dbdgemm(int fold, char Order, char TransA, char TransB, int M, int N, int K,
+                 double alpha, const double *A, int lda, const double *B, int ldb,
+                 double *C, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

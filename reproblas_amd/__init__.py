@@ -1,0 +1,370 @@
+"""reproblas_amd -- reproducible (binned) FP64 BLAS on AMD Instinct MI355X.
+
+Python mirror of the reproBLAS C API (reference include/reproBLAS.h,
+include/binnedBLAS.h) over the C ABI of ``libreproblas_hip.so``. Every
+reduction runs in the library's gfx950 kernels; this module only marshals
+arguments. Operands may be numpy arrays (staged to the GPU by the library) or
+torch tensors on a ROCm device (used in place). There is no CPU compute path:
+if the library is missing the import of any compute function fails loudly.
+
+    import reproblas_amd as rb
+    s = rb.rdsum(x)                    # fold 3, like reproBLAS_dsum
+    d = rb.rddot(x, y, fold=2)         # like reproBLAS_rddot(2, ...)
+    rb.rdgemm('N', 'N', 1.0, A, B, 0.0, C)   # column-major, in place on C
+"""
+import ctypes
+import os
+
+__all__ = [
+    "lib", "DEFAULT_FOLD", "ReproBLASError",
+    "rdsum", "rdasum", "rdnrm2", "rddot", "rdgemv", "rdgemm",
+    "dsum", "dasum", "dnrm2", "ddot", "dgemv", "dgemm",
+    "dbdsum", "dbdasum", "dbddot", "dbdssq", "dbdgemv", "dbdgemm", "damax", "damaxm",
+    "dbdconv", "ddbconv", "dbdbadd", "dbdbaddv", "dbsetzero", "dbnum",
+    "gemm_tilings", "set_gemm_tiling", "get_gemm_tiling",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libreproblas_hip.so")
+DEFAULT_FOLD = 3  # reference config.h:14 (DIDEFAULTFOLD)
+
+
+class ReproBLASError(RuntimeError):
+    pass
+
+
+_lib = None
+_c_int, _c_i64, _c_double, _c_char, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_char, ctypes.c_void_p
+
+_SIGS = {
+    # reproBLAS.h
+    "reproBLAS_rdsum": (_c_double, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rdasum": (_c_double, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rdnrm2": (_c_double, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rddot": (_c_double, [_c_int, _c_int, _vp, _c_int, _vp, _c_int]),
+    "reproBLAS_rdgemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _c_double, _vp, _c_int, _vp, _c_int,
+                                _c_double, _vp, _c_int]),
+    "reproBLAS_rdgemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _c_double, _vp, _c_int,
+                                _vp, _c_int, _c_double, _vp, _c_int]),
+    "reproBLAS_dsum": (_c_double, [_c_int, _vp, _c_int]),
+    "reproBLAS_dasum": (_c_double, [_c_int, _vp, _c_int]),
+    "reproBLAS_dnrm2": (_c_double, [_c_int, _vp, _c_int]),
+    "reproBLAS_ddot": (_c_double, [_c_int, _vp, _c_int, _vp, _c_int]),
+    "reproBLAS_dgemv": (None, [_c_char, _c_char, _c_int, _c_int, _c_double, _vp, _c_int, _vp, _c_int, _c_double,
+                               _vp, _c_int]),
+    "reproBLAS_dgemm": (None, [_c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _c_double, _vp, _c_int, _vp,
+                               _c_int, _c_double, _vp, _c_int]),
+    # binnedBLAS.h
+    "binnedBLAS_damax": (_c_double, [_c_int, _vp, _c_int]),
+    "binnedBLAS_damaxm": (_c_double, [_c_int, _vp, _c_int, _vp, _c_int]),
+    "binnedBLAS_dbdsum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
+    "binnedBLAS_dbdasum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
+    "binnedBLAS_dbddot": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp]),
+    "binnedBLAS_dbdssq": (_c_double, [_c_int, _c_int, _vp, _c_int, _c_double, _vp]),
+    "binnedBLAS_dbdgemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _c_double, _vp, _c_int, _vp, _c_int,
+                                  _vp, _c_int]),
+    "binnedBLAS_dbdgemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _c_double, _vp, _c_int,
+                                  _vp, _c_int, _vp, _c_int]),
+    # binned.h (subset used from Python)
+    "binned_dbnum": (_c_int, [_c_int]),
+    "binned_dbsetzero": (None, [_c_int, _vp]),
+    "binned_dbdconv": (None, [_c_int, _c_double, _vp]),
+    "binned_ddbconv": (_c_double, [_c_int, _vp]),
+    "binned_dbdbadd": (None, [_c_int, _vp, _vp]),
+    "binned_dbdbaddv": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int]),
+    # reproblas_hip.h
+    "reproblas_hip_last_error": (_c_int, []),
+    "reproblas_hip_last_error_string": (ctypes.c_char_p, []),
+    "reproblas_hip_clear_error": (None, []),
+    "reproblas_hip_set_stream": (None, [_vp]),
+    "reproblas_hip_get_stream": (_vp, []),
+    "reproblas_hip_min_fold": (_c_int, []),
+    "reproblas_hip_max_fold": (_c_int, []),
+    "reproblas_hip_rdsum_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_rdasum_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_rdnrm2_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_rddot_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_dbdsum_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_dbdasum_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_dbddot_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_dbdssq_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp, _vp]),
+    "reproblas_hip_rdgemv_device": (_c_int, [_c_int, _c_char, _c_char, _c_i64, _c_i64, _c_double, _vp, _c_i64, _vp,
+                                             _c_i64, _c_double, _vp, _c_i64, _vp]),
+    "reproblas_hip_rdgemm_device": (_c_int, [_c_int, _c_char, _c_char, _c_char, _c_i64, _c_i64, _c_i64, _c_double,
+                                             _vp, _c_i64, _vp, _c_i64, _c_double, _vp, _c_i64, _vp]),
+    "reproblas_hip_gemm_tiling_count": (_c_int, []),
+    "reproblas_hip_gemm_set_tiling": (_c_int, [_c_int]),
+    "reproblas_hip_gemm_get_tiling": (_c_int, []),
+}
+
+
+def lib():
+    """The loaded C-ABI library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ReproBLASError(
+                f"{LIB_PATH} is not built: run `python -m reproblas_amd.build` (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+# ------------------------------------------------------------- marshalling --
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a):
+    """Address of the first element of a numpy array or torch tensor."""
+    if a is None:
+        return None
+    if _is_torch(a):
+        if str(a.dtype) != "torch.float64":
+            raise TypeError("reproblas_amd works on float64 tensors")
+        if not a.is_contiguous():
+            raise ValueError("reproblas_amd needs contiguous tensors (pass strides through inc/ld arguments)")
+        return ctypes.c_void_p(a.data_ptr())
+    import numpy as np
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64:
+        raise TypeError("reproblas_amd works on float64 numpy arrays or torch tensors")
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("reproblas_amd needs contiguous arrays (pass strides through inc/ld arguments)")
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _bind_stream(*arrays):
+    """Route the synchronous calls onto torch's current stream when operands are torch tensors."""
+    for a in arrays:
+        if a is not None and _is_torch(a) and a.is_cuda:
+            import torch
+            lib().reproblas_hip_set_stream(ctypes.c_void_p(torch.cuda.current_stream(a.device).cuda_stream))
+            return
+    lib().reproblas_hip_set_stream(None)
+
+
+def _check():
+    L = lib()
+    e = L.reproblas_hip_last_error()
+    if e != 0:
+        msg = L.reproblas_hip_last_error_string().decode()
+        L.reproblas_hip_clear_error()
+        raise ReproBLASError(f"reproblas_hip error {e}: {msg}")
+
+
+def _numel(a):
+    return a.numel() if _is_torch(a) else a.size
+
+
+def _n_of(a, inc):
+    inc = abs(inc) or 1
+    return (_numel(a) + inc - 1) // inc
+
+
+# --------------------------------------------------------------------- BLAS1 --
+def rdsum(x, fold=DEFAULT_FOLD, inc=1, n=None):
+    """reproBLAS_rdsum: reproducible sum of x[0], x[inc], ... (n terms)."""
+    _bind_stream(x)
+    r = lib().reproBLAS_rdsum(fold, _n_of(x, inc) if n is None else n, _ptr(x), inc)
+    _check()
+    return r
+
+
+def rdasum(x, fold=DEFAULT_FOLD, inc=1, n=None):
+    """reproBLAS_rdasum: reproducible sum of |x_i|."""
+    _bind_stream(x)
+    r = lib().reproBLAS_rdasum(fold, _n_of(x, inc) if n is None else n, _ptr(x), inc)
+    _check()
+    return r
+
+
+def rdnrm2(x, fold=DEFAULT_FOLD, inc=1, n=None):
+    """reproBLAS_rdnrm2: reproducible Euclidean norm."""
+    _bind_stream(x)
+    r = lib().reproBLAS_rdnrm2(fold, _n_of(x, inc) if n is None else n, _ptr(x), inc)
+    _check()
+    return r
+
+
+def rddot(x, y, fold=DEFAULT_FOLD, incx=1, incy=1, n=None):
+    """reproBLAS_rddot: reproducible sum of fl(x_i*y_i)."""
+    _bind_stream(x, y)
+    if n is None:
+        n = min(_n_of(x, incx), _n_of(y, incy))
+    r = lib().reproBLAS_rddot(fold, n, _ptr(x), incx, _ptr(y), incy)
+    _check()
+    return r
+
+
+def dsum(x, inc=1):
+    return rdsum(x, DEFAULT_FOLD, inc)
+
+
+def dasum(x, inc=1):
+    return rdasum(x, DEFAULT_FOLD, inc)
+
+
+def dnrm2(x, inc=1):
+    return rdnrm2(x, DEFAULT_FOLD, inc)
+
+
+def ddot(x, y, incx=1, incy=1):
+    return rddot(x, y, DEFAULT_FOLD, incx, incy)
+
+
+# ------------------------------------------------------- binned-state BLAS1 --
+def dbnum(fold=DEFAULT_FOLD):
+    return 2 * fold
+
+
+def dbdsum(Y, x, fold=DEFAULT_FOLD, inc=1):
+    """binnedBLAS_dbdsum: Y (a 2*fold float64 binned value) += sum(x)."""
+    _bind_stream(x, Y)
+    lib().binnedBLAS_dbdsum(fold, _n_of(x, inc), _ptr(x), inc, _ptr(Y))
+    _check()
+
+
+def dbdasum(Y, x, fold=DEFAULT_FOLD, inc=1):
+    _bind_stream(x, Y)
+    lib().binnedBLAS_dbdasum(fold, _n_of(x, inc), _ptr(x), inc, _ptr(Y))
+    _check()
+
+
+def dbddot(Z, x, y, fold=DEFAULT_FOLD, incx=1, incy=1):
+    _bind_stream(x, y, Z)
+    n = min(_n_of(x, incx), _n_of(y, incy))
+    lib().binnedBLAS_dbddot(fold, n, _ptr(x), incx, _ptr(y), incy, _ptr(Z))
+    _check()
+
+
+def dbdssq(Y, x, scale=0.0, fold=DEFAULT_FOLD, inc=1):
+    """binnedBLAS_dbdssq: returns the new scale; Y accumulates the scaled squares."""
+    _bind_stream(x, Y)
+    r = lib().binnedBLAS_dbdssq(fold, _n_of(x, inc), _ptr(x), inc, scale, _ptr(Y))
+    _check()
+    return r
+
+
+def damax(x, inc=1):
+    _bind_stream(x)
+    r = lib().binnedBLAS_damax(_n_of(x, inc), _ptr(x), inc)
+    _check()
+    return r
+
+
+def damaxm(x, y, incx=1, incy=1):
+    _bind_stream(x, y)
+    r = lib().binnedBLAS_damaxm(min(_n_of(x, incx), _n_of(y, incy)), _ptr(x), incx, _ptr(y), incy)
+    _check()
+    return r
+
+
+# --------------------------------------------------------- binned helpers --
+def dbsetzero(Y, fold=DEFAULT_FOLD):
+    lib().binned_dbsetzero(fold, _ptr(Y))
+
+
+def dbdconv(x, fold=DEFAULT_FOLD):
+    import numpy as np
+    Y = np.zeros(2 * fold)
+    lib().binned_dbdconv(fold, x, _ptr(Y))
+    return Y
+
+
+def ddbconv(Y, fold=DEFAULT_FOLD):
+    return lib().binned_ddbconv(fold, _ptr(Y))
+
+
+def dbdbadd(X, Y, fold=DEFAULT_FOLD):
+    lib().binned_dbdbadd(fold, _ptr(X), _ptr(Y))
+
+
+def dbdbaddv(X, Y, n, fold=DEFAULT_FOLD, incx=1, incy=1):
+    _bind_stream(X, Y)
+    lib().binned_dbdbaddv(fold, n, _ptr(X), incx, _ptr(Y), incy)
+    _check()
+
+
+# ------------------------------------------------------------------ level 2/3 --
+def _ch(c):
+    return c.encode() if isinstance(c, str) else c
+
+
+def rdgemv(trans, alpha, A, x, beta, y, fold=DEFAULT_FOLD, order="C", M=None, N=None, lda=None, incx=1, incy=1):
+    """reproBLAS_rdgemv on a 2-D A; y is updated in place.
+
+    ``order`` 'C' means column-major (the reference's default for anything but
+    'r'/'R'), so a C-contiguous 2-D array of shape (n, m) is an m x n
+    column-major matrix. M, N, lda default from that view."""
+    rm = order in ("r", "R")
+    if M is None or N is None or lda is None:
+        sh = tuple(A.shape)
+        if rm:
+            M, N, lda = sh[0], sh[1], sh[1]
+        else:
+            M, N, lda = sh[1], sh[0], sh[1]
+    _bind_stream(A, x, y)
+    lib().reproBLAS_rdgemv(fold, _ch(order), _ch(trans), M, N, alpha, _ptr(A), lda, _ptr(x), incx, beta, _ptr(y),
+                           incy)
+    _check()
+
+
+def rdgemm(transa, transb, alpha, A, B, beta, C, fold=DEFAULT_FOLD, order="C", M=None, N=None, K=None,
+           lda=None, ldb=None, ldc=None):
+    """reproBLAS_rdgemm; C is updated in place. Shapes as for :func:`rdgemv`."""
+    rm = order in ("r", "R")
+
+    def dims(X):
+        s = tuple(X.shape)
+        return (s[0], s[1], s[1]) if rm else (s[1], s[0], s[1])
+
+    ra, ca, la = dims(A)
+    rb_, cb, lb = dims(B)
+    rc, cc, lc = dims(C)
+    na = transa in ("n", "N")
+    M = (ra if na else ca) if M is None else M
+    K = (ca if na else ra) if K is None else K
+    N = (cb if transb in ("n", "N") else rb_) if N is None else N
+    lda, ldb, ldc = lda or la, ldb or lb, ldc or lc
+    _bind_stream(A, B, C)
+    lib().reproBLAS_rdgemm(fold, _ch(order), _ch(transa), _ch(transb), M, N, K, alpha, _ptr(A), lda, _ptr(B), ldb,
+                           beta, _ptr(C), ldc)
+    _check()
+
+
+def dgemv(trans, alpha, A, x, beta, y, **kw):
+    return rdgemv(trans, alpha, A, x, beta, y, fold=DEFAULT_FOLD, **kw)
+
+
+def dgemm(transa, transb, alpha, A, B, beta, C, **kw):
+    return rdgemm(transa, transb, alpha, A, B, beta, C, fold=DEFAULT_FOLD, **kw)
+
+
+def dbdgemv(fold, order, trans, M, N, alpha, A, lda, x, incx, Yb, incy=1):
+    _bind_stream(A, x, Yb)
+    lib().binnedBLAS_dbdgemv(fold, _ch(order), _ch(trans), M, N, alpha, _ptr(A), lda, _ptr(x), incx, _ptr(Yb), incy)
+    _check()
+
+
+def dbdgemm(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, Cb, ldc):
+    _bind_stream(A, B, Cb)
+    lib().binnedBLAS_dbdgemm(fold, _ch(order), _ch(transa), _ch(transb), M, N, K, alpha, _ptr(A), lda, _ptr(B), ldb,
+                             _ptr(Cb), ldc)
+    _check()
+
+
+def gemm_tilings():
+    return lib().reproblas_hip_gemm_tiling_count()
+
+
+def set_gemm_tiling(t):
+    if lib().reproblas_hip_gemm_set_tiling(t) != 0:
+        _check()
+
+
+def get_gemm_tiling():
+    return lib().reproblas_hip_gemm_get_tiling()

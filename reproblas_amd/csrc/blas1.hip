@@ -1,0 +1,496 @@
+// blas1.hip -- single-pass binned reductions over device vectors:
+// sum, asum, dot and the scaled sum of squares behind nrm2.
+//
+// Reference path: binnedBLAS_dmdsum (src/binnedBLAS/dmdsum.ccog:57-89),
+// dmdasum (dmdasum.ccog:57-85), dmddot (dmddot.ccog:59-88) and dmdssq
+// (dmdssq.ccog:60-101), merged the way binned_dbdbadd does
+// (src/binned/dmdmadd.c:27-74), converted by binned_ddmconv.
+//
+// Layout of the work. The vector is cut into wave tiles of 64*U terms. Each
+// wave walks tiles grid-stride. Every lane keeps its own fold-F primaries and
+// carries in VGPRs; the index (and, for nrm2, the scale) is common to the
+// wave. Per tile a lane takes the max |term| of its U terms; only when some
+// lane's max is no longer admissible at the wave's index does the wave take
+// the slow path (wave max, raise the index). Then every lane deposits its U
+// terms (the reference's deposit, element by element) and the wave
+// renormalises every 2048 deposits. At the end the 64 lanes merge by a
+// butterfly of exact same-index adds, the waves of a workgroup merge in LDS,
+// every workgroup writes one partial, and the last workgroup to finish (a
+// ticket counter) merges all partials, folds in the caller's binned value and
+// converts. HBM is read exactly once.
+//
+// Why this equals the reference bit for bit: the slice each bin receives from
+// a term is independent of order and of the accumulator, the index ends at
+// dindex(max |term|) either way, and the merges are exact.
+//
+// Non-finite terms poison a lane's top primary (the "|1" of an infinity is a
+// NaN). A wave that ends poisoned re-reads its own tiles and replaces its top
+// primary by the class of its terms (NaN if any NaN or both infinities, else
+// the infinity); merges then combine classes by plain addition, as the
+// reference's infinity path does.
+#include <float.h>
+#include <limits.h>
+
+#include "binned_core.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace rb {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+struct Blas1Args {
+  int64_t n;
+  const double* x;
+  int64_t incx;
+  const double* y;
+  int64_t incy;
+  int64_t tiles;       // wave tiles covering n
+  int64_t full_tiles;  // tiles with no out-of-range element
+  double* partials;    // gridDim.x records of (2*fold+1) doubles
+  unsigned* ticket;
+  const double* state_in;
+  double scale_in;
+  const double* scale_in_dev;  // when set, the scale of state_in is read here
+  double* state_out;
+  double* scale_out;
+  double* result;
+  int fold;
+};
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
+
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double uniform_double(double v) {
+  const uint64_t u = bits(v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return from_bits(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, shfl_xor_d(v, m));
+  return v;
+}
+
+// The term as the reference's infinity path adds it (dmdsum.ccog:68-72,
+// dmdasum and dmdssq use fabs, dmddot the product).
+template <int T>
+__device__ __forceinline__ double plain_term(double x, double y) {
+  if (T == kSum) return x;
+  if (T == kDot) return x * y;
+  return fabs(x);
+}
+
+// Loads the U raw terms of tile t for this lane. MODE 0: unit stride with
+// 16-byte vector loads (operands 16-byte aligned); MODE 1: unit stride, 8-byte
+// loads; MODE 2: arbitrary stride. Out-of-range terms read as 0, which every
+// deposit leaves unchanged.
+template <int T, int U, int MODE>
+__device__ __forceinline__ void load_tile(const Blas1Args& a, int64_t t, int lane, bool full, double (&vx)[U],
+                                          double (&vy)[U]) {
+  const int64_t base = t * (64 * U);
+  if (MODE == 0 && full) {
+    const double2* xv = reinterpret_cast<const double2*>(a.x + base);
+#pragma unroll
+    for (int k = 0; k < U / 2; ++k) {
+      const double2 d = xv[k * 64 + lane];
+      vx[2 * k] = d.x;
+      vx[2 * k + 1] = d.y;
+    }
+    if (T == kDot) {
+      const double2* yv = reinterpret_cast<const double2*>(a.y + base);
+#pragma unroll
+      for (int k = 0; k < U / 2; ++k) {
+        const double2 d = yv[k * 64 + lane];
+        vy[2 * k] = d.x;
+        vy[2 * k + 1] = d.y;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const int64_t e = base + k * 64 + lane;
+    const bool in = full || e < a.n;
+    const int64_t ix = (MODE == 2) ? e * a.incx : e;
+    vx[k] = in ? a.x[ix] : 0.0;
+    if (T == kDot) {
+      const int64_t iy = (MODE == 2) ? e * a.incy : e;
+      vy[k] = in ? a.y[iy] : 0.0;
+    }
+  }
+}
+
+template <int CAP>
+struct State {
+  double p[CAP];
+  double c[CAP];
+  double scl;  // nrm2 only
+};
+
+template <int CAP>
+__device__ __forceinline__ void store_state(double* dst, int fold, const State<CAP>& s) {
+#pragma unroll
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) {
+      dst[j] = s.p[j];
+      dst[fold + j] = s.c[j];
+    }
+  dst[2 * fold] = s.scl;
+}
+
+template <int CAP>
+__device__ __forceinline__ void load_state(const double* src, int fold, State<CAP>& s) {
+#pragma unroll
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) {
+      s.p[j] = src[j];
+      s.c[j] = src[fold + j];
+    }
+  s.scl = src[2 * fold];
+}
+
+// General merge of two renormalised values (with scales for nrm2).
+template <int CAP, int T>
+__device__ __forceinline__ void merge(int fold, State<CAP>& y, const State<CAP>& x) {
+  if (T == kSSq) {
+    if (nan_or_inf(x.p[0]) || nan_or_inf(y.p[0])) {
+      // classes combine by addition; the scale no longer matters
+      if (!nan_or_inf(y.p[0])) {
+#pragma unroll
+        for (int j = 0; j < CAP; ++j) { y.p[j] = x.p[j]; y.c[j] = x.c[j]; }
+        y.scl = x.scl;
+      } else if (nan_or_inf(x.p[0])) {
+        y.p[0] += x.p[0];
+      }
+      return;
+    }
+    y.scl = add_scaled(fold, x.scl, x.p, x.c, y.scl, y.p, y.c);
+  } else {
+    add(fold, y.p, y.c, x.p, x.c);
+  }
+}
+
+// Merge the 64 lanes of a wave (general add: lanes may differ in index).
+template <int CAP, int T>
+__device__ __forceinline__ void wave_merge_general(int fold, State<CAP>& s) {
+#pragma unroll
+  for (int m = 1; m <= 32; m <<= 1) {
+    State<CAP> o;
+#pragma unroll
+    for (int j = 0; j < CAP; ++j) {
+      o.p[j] = shfl_xor_d(s.p[j], m);
+      o.c[j] = shfl_xor_d(s.c[j], m);
+    }
+    o.scl = shfl_xor_d(s.scl, m);
+    // lower lane of each pair keeps (y = lower, x = upper) so both lanes agree
+    merge<CAP, T>(fold, s, o);
+  }
+}
+
+// Class of the terms of the tiles this wave walked: NaN if any term is NaN or
+// both infinities occur, else the infinity that occurs. Terms as the
+// reference's infinity path forms them.
+template <int T, int U, int MODE>
+__device__ double wave_term_class(const Blas1Args& a, int64_t first, int64_t step, int lane) {
+  bool nan = false, pinf = false, ninf = false;
+  for (int64_t t = first; t < a.tiles; t += step) {
+    double vx[U], vy[U];
+    load_tile<T, U, MODE>(a, t, lane, t < a.full_tiles, vx, vy);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const double v = plain_term<T>(vx[k], T == kDot ? vy[k] : 0.0);
+      nan |= (v != v);
+      pinf |= (v == __builtin_inf());
+      ninf |= (v == -__builtin_inf());
+    }
+  }
+  const bool any_nan = __any(nan), any_p = __any(pinf), any_n = __any(ninf);
+  if (any_nan || (any_p && any_n) || !(any_p || any_n)) return __builtin_nan("");
+  return any_p ? __builtin_inf() : -__builtin_inf();
+}
+
+template <int CAP, bool FIXED, int T, int U, int MODE>
+__global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + wave;
+  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+
+  // Every wave starts at index 51, the reference's representation of an
+  // empty sum (dmdupdate of 0.0 on a zero value), so every tile deposits.
+  State<CAP> s;
+  int idx = kMaxIndex;
+  set_bins(fold, idx, s.p, s.c);
+  double lim = admit_limit(idx);
+  double scl = 0.0, inv = 0.0;  // nrm2: current scale of the wave and 1/scale
+  int dep = 0;
+
+  for (int64_t t = wave_gid; t < a.tiles; t += nwaves) {
+    double vx[U], vy[U];
+    load_tile<T, U, MODE>(a, t, lane, t < a.full_tiles, vx, vy);
+    double v[U];
+    double m = 0.0;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (T == kSum) v[k] = vx[k];
+      if (T == kASum) v[k] = fabs(vx[k]);
+      if (T == kDot) v[k] = vx[k] * vy[k];
+      if (T == kSSq) v[k] = vx[k];
+      m = fmax(m, fabs(v[k]));
+    }
+    bool need;
+    if (T == kSSq) {
+      const double ms = m * inv;
+      need = (dscale(m) > scl) || (ms * ms >= lim);
+    } else {
+      need = (m >= lim);
+    }
+    // the endurance renorm comes before the update as in the reference drivers
+    if (dep + U > kEndurance) {
+      renorm(fold, s.p, s.c);
+      dep = 0;
+    }
+    if (__any(need)) {
+      double M = uniform_double(wave_max(m));
+      if (T == kSSq) {
+        const double ns = dscale(M);
+        if (ns > scl) {
+          if (scl > 0.0) {
+            rescale(fold, ns, scl, s.p, s.c);
+            idx = uniform_int(dmindex(s.p[0]));
+          }
+          scl = ns;
+          inv = 1.0 / scl;
+        }
+        M = M / scl;  // reference dmdssq.ccog:98-99
+        M = M * M;
+      }
+      const int nI = dindex(M);
+      if (nI < idx) {
+        raise_index(fold, idx - nI, nI, s.p, s.c);
+        idx = nI;
+      }
+      lim = admit_limit(idx);
+    }
+    if (idx == 0) {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        double tv = v[k];
+        if (T == kSSq) { tv = v[k] * inv; tv = tv * tv; }
+        deposit(fold, true, tv, s.p);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        double tv = v[k];
+        if (T == kSSq) { tv = v[k] * inv; tv = tv * tv; }
+        deposit(fold, false, tv, s.p);
+      }
+    }
+    dep += U;
+  }
+  renorm(fold, s.p, s.c);
+  s.scl = scl;
+
+  // Lanes share the index (and scale): exact same-index butterfly.
+#pragma unroll
+  for (int m = 1; m <= 32; m <<= 1) {
+    double op[CAP], oc[CAP];
+#pragma unroll
+    for (int j = 0; j < CAP; ++j) {
+      op[j] = shfl_xor_d(s.p[j], m);
+      oc[j] = shfl_xor_d(s.c[j], m);
+    }
+    if (!nan_or_inf(s.p[0]) && !nan_or_inf(op[0])) {
+      add_same_index(fold, idx, s.p, s.c, op, oc);
+    } else {
+      s.p[0] = s.p[0] + op[0];  // keep the poison visible
+    }
+  }
+  if (nan_or_inf(uniform_double(s.p[0]))) {
+    s.p[0] = wave_term_class<T, U, MODE>(a, wave_gid, nwaves, lane);
+  }
+
+  // Workgroup merge through LDS, then one partial per workgroup.
+  __shared__ double lds[kWaves][2 * CAP + 1];
+  __shared__ bool last;
+  if (lane == 0) store_state(lds[wave], fold, s);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    State<CAP> acc, o;
+    load_state(lds[0], fold, acc);
+    for (int w = 1; w < kWaves; ++w) {
+      load_state(lds[w], fold, o);
+      merge<CAP, T>(fold, acc, o);
+    }
+    store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, acc);
+    __threadfence();
+    const unsigned ticket = atomicAdd(a.ticket, 1u);
+    last = (ticket == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+
+  // Last workgroup: merge every partial (exact, so the order is free; it is
+  // fixed anyway), fold in the caller's value, convert.
+  __threadfence();
+  State<CAP> acc;
+  bool have = false;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
+    State<CAP> o;
+    const double* src = a.partials + (int64_t)b * (2 * fold + 1);
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        o.p[j] = __builtin_nontemporal_load(src + j);
+        o.c[j] = __builtin_nontemporal_load(src + fold + j);
+      }
+    o.scl = __builtin_nontemporal_load(src + 2 * fold);
+    if (!have) {
+      acc = o;
+      have = true;
+    } else {
+      merge<CAP, T>(fold, acc, o);
+    }
+  }
+  if (!have) {
+    set_bins(fold, kMaxIndex, acc.p, acc.c);
+    acc.scl = 0.0;
+  }
+  wave_merge_general<CAP, T>(fold, acc);
+  if (lane == 0) store_state(lds[wave], fold, acc);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    State<CAP> o;
+    load_state(lds[0], fold, acc);
+    for (int w = 1; w < kWaves; ++w) {
+      load_state(lds[w], fold, o);
+      merge<CAP, T>(fold, acc, o);
+    }
+    // caller's value, renormalised first (the reference ends with a renorm)
+    State<CAP> y;
+    if (a.state_in) {
+#pragma unroll
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          y.p[j] = a.state_in[j];
+          y.c[j] = a.state_in[fold + j];
+        }
+      renorm(fold, y.p, y.c);
+    } else {
+      set_zero(fold, y.p, y.c);
+    }
+    y.scl = a.scale_in_dev ? *a.scale_in_dev : a.scale_in;
+    if (a.n == 0) {
+      // nothing deposited: the caller's value only (the reference's final renorm)
+    } else if (y.p[0] == 0.0) {
+      y = acc;
+    } else {
+      merge<CAP, T>(fold, y, acc);
+    }
+    renorm(fold, y.p, y.c);
+    double scale = y.scl;
+    if (T == kSSq && nan_or_inf(y.p[0])) scale = 1.0;  // dscale(1.0), dmdssq.ccog tail
+    if (a.state_out) {
+#pragma unroll
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          a.state_out[j] = y.p[j];
+          a.state_out[fold + j] = y.c[j];
+        }
+    }
+    if (a.scale_out) *a.scale_out = scale;
+    if (a.result) {
+      const double v = to_double(fold, y.p, y.c);
+      *a.result = (T == kSSq) ? scale * sqrt(v) : v;
+    }
+    *a.ticket = 0u;  // ready for the next launch on this stream
+  }
+}
+
+template <int CAP, bool FIXED, int T, int U>
+bool launch_mode(int mode, const Blas1Args& a, int blocks, hipStream_t s) {
+  switch (mode) {
+    case 0: blas1_kernel<CAP, FIXED, T, U, 0><<<blocks, kThreads, 0, s>>>(a); break;
+    case 1: blas1_kernel<CAP, FIXED, T, U, 1><<<blocks, kThreads, 0, s>>>(a); break;
+    default: blas1_kernel<CAP, FIXED, T, U, 2><<<blocks, kThreads, 0, s>>>(a); break;
+  }
+  return RB_HIP(hipGetLastError());
+}
+
+template <int CAP, bool FIXED>
+bool launch_term(Term term, int mode, const Blas1Args& a, int blocks, hipStream_t s) {
+  switch (term) {
+    case kSum: return launch_mode<CAP, FIXED, kSum, 8>(mode, a, blocks, s);
+    case kASum: return launch_mode<CAP, FIXED, kASum, 8>(mode, a, blocks, s);
+    case kDot: return launch_mode<CAP, FIXED, kDot, 8>(mode, a, blocks, s);
+    default: return launch_mode<CAP, FIXED, kSSq, 8>(mode, a, blocks, s);
+  }
+}
+
+constexpr int kTileU = 8;
+
+}  // namespace
+
+bool fold_supported(int fold) { return fold >= 2 && fold <= 8; }
+
+bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
+           const double* state_in, double scale_in, const double* scale_in_dev, double* state_out, double* scale_out,
+           double* result, hipStream_t s) {
+  if (!fold_supported(fold)) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    return false;
+  }
+  const DeviceInfo* dev = device_info();
+  if (!dev) return false;
+  Blas1Args a;
+  a.n = n < 0 ? 0 : n;
+  a.x = x;
+  a.incx = incx;
+  a.y = y;
+  a.incy = incy;
+  const int64_t per_tile = 64 * kTileU;
+  a.tiles = (a.n + per_tile - 1) / per_tile;
+  a.full_tiles = a.n / per_tile;
+  a.state_in = state_in;
+  a.scale_in = scale_in;
+  a.scale_in_dev = scale_in_dev;
+  a.state_out = state_out;
+  a.scale_out = scale_out;
+  a.result = result;
+  a.fold = fold;
+
+  // grid: enough waves to keep every CU streaming, at most one wave per tile
+  int64_t blocks = (a.tiles + kWaves - 1) / kWaves;
+  const int64_t cap = (int64_t)dev->cus * 8;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  a.partials = (double*)workspace(s, kSlotPartials, sizeof(double) * (size_t)blocks * (2 * fold + 1));
+  a.ticket = tickets(s, 1);
+  if (!a.partials || !a.ticket) return false;
+
+  int mode;
+  const bool unit = (incx == 1) && (term != kDot || incy == 1);
+  if (!unit) {
+    mode = 2;
+  } else {
+    const bool aligned = ((uintptr_t)x % 16 == 0) && (term != kDot || (uintptr_t)y % 16 == 0);
+    mode = aligned ? 0 : 1;
+  }
+  switch (fold) {
+    case 2: return launch_term<2, true>(term, mode, a, (int)blocks, s);
+    case 3: return launch_term<3, true>(term, mode, a, (int)blocks, s);
+    case 4: return launch_term<4, true>(term, mode, a, (int)blocks, s);
+    default: return launch_term<8, false>(term, mode, a, (int)blocks, s);
+  }
+}
+
+}  // namespace rb

@@ -1,0 +1,426 @@
+// gemm.hip -- binned gemm on device operands.
+//
+// Reference: binnedBLAS_dbdgemm (src/binnedBLAS/dbdgemm.ccog:61-420) under
+// reproBLAS_rdgemm (src/reproBLAS/rdgemm.c:41-105):
+//   C_ij = ddbconv( dbdconv(beta*C_ij) (+) sum_k fl(fl(alpha*A_ik) * B_kj) )
+// alpha is applied to A (every branch of dbdgemm.ccog), beta == 0 ignores C,
+// beta == 1 converts C unscaled. Every output is one binned dot of K terms,
+// so the tile shape and the chunking of K are free.
+//
+// Work layout (FP64 VALU bound; MFMA is not used because its fused
+// accumulation would round products and partial sums differently):
+//   * a 256-thread workgroup owns a (16*TM) x (16*TN) tile of C; each lane
+//     owns TM x TN outputs, each a fold-F binned value: primaries in VGPRs,
+//     carries as int32 in VGPRs (a carry changes only at a renorm, by -2..1);
+//   * K is walked in chunks of KC = 32: the chunk of A (times alpha) and B is
+//     staged in LDS, then every lane runs the deposit of all TM*TN products
+//     for each k -- exactly 1 mul + (4F-2) add/or per product;
+//   * an output's index must equal dindex(max_k |fl(a*b)|). Per chunk a lane
+//     compares a bound fl(fl(|alpha| max|a|) max|b|) (row and column maxima
+//     from a small pre-pass) with the admissible limit of its outputs; only
+//     when the bound reaches it does the lane take the exact per-product max
+//     of the chunk and raise indices. Both give the reference's index.
+//   * renormalisation every 2048 deposits; the epilogue folds in beta*C (or
+//     the caller's binned C), converts and writes.
+// Non-finite products poison an output's top primary; the epilogue then
+// recomputes that output's class from its K terms (NaN, +inf or -inf).
+#include "binned_core.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace rb {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int KC = 32;
+
+struct GemmArgs {
+  int64_t M, N, K;
+  const double* A;
+  int64_t sai, sak;  // A(i,k) = A[i*sai + k*sak]
+  const double* B;
+  int64_t sbk, sbj;  // B(k,j) = B[k*sbk + j*sbj]
+  double alpha;
+  double* C;  // double output (when Cb is null)
+  int64_t sci, scj;
+  double beta;
+  double* Cb;  // binned output accumulated into, element (i,j) at (i*sbi + j*sbj2)*2F
+  int64_t cbi, cbj;
+  const double* rowmax;  // [chunks][M] max |A(i,k)| over the chunk
+  const double* colmax;  // [chunks][N] max |B(k,j)| over the chunk
+  int fold;
+};
+
+// ---- int-carry variants of renorm / raise (carries are small integers) ----
+template <int CAP>
+__device__ __forceinline__ void renorm_ic(int fold, double (&p)[CAP], int (&c)[CAP]) {
+  if (p[0] == 0.0 || nan_or_inf(p[0])) return;
+#pragma unroll
+  for (int j = 0; j < CAP; ++j) {
+    if (j < fold) {
+      // bits 51:50 of the mantissa live in bits 19:18 of the high word
+      uint32_t hi = (uint32_t)(bits(p[j]) >> 32);
+      c[j] += (int)((hi >> 18) & 3u) - 2;
+      hi = (hi & ~(1u << 18)) | (1u << 19);
+      p[j] = from_bits(((uint64_t)hi << 32) | (bits(p[j]) & 0xffffffffull));
+    }
+  }
+}
+
+template <int CAP>
+__device__ __forceinline__ void raise_ic(int fold, int shift, int I, double (&p)[CAP], int (&c)[CAP]) {
+  if (shift <= 0) return;
+  double np[CAP];
+  int nc[CAP];
+#pragma unroll
+  for (int j = 0; j < CAP; ++j) {
+    if (j < fold) {
+      if (j >= shift) {
+        double v = 0.0;
+        int w = 0;
+#pragma unroll
+        for (int t = 0; t < CAP; ++t)
+          if (t == j - shift) { v = p[t]; w = c[t]; }
+        np[j] = v;
+        nc[j] = w;
+      } else {
+        np[j] = bin(I + j);
+        nc[j] = 0;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) { p[j] = np[j]; c[j] = nc[j]; }
+}
+
+// ---------------------------------------------------------------- prepass --
+// rowmax[ch][i] = max_{k in chunk ch} |A(i,k)|, colmax[ch][j] likewise for B.
+__global__ __launch_bounds__(256) void chunk_max_kernel(int64_t rows, int64_t K, const double* X, int64_t s_row,
+                                                        int64_t s_k, double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t chunks = (K + KC - 1) / KC;
+  if (t >= rows * chunks) return;
+  const int64_t r = t % rows, ch = t / rows;
+  const int64_t k0 = ch * KC;
+  const int64_t k1 = (k0 + KC < K) ? k0 + KC : K;
+  double m = 0.0;
+  for (int64_t k = k0; k < k1; ++k) m = fmax(m, fabs(X[r * s_row + k * s_k]));
+  out[ch * rows + r] = m;
+}
+
+template <int CAP>
+__device__ __forceinline__ void beta_init(int fold, double beta, double yv, double (&p)[CAP], double (&c)[CAP]) {
+  set_zero(fold, p, c);
+  if (beta == 0.0) return;
+  const double v = (beta == 1.0) ? yv : yv * beta;
+  update(fold, v, p, c);
+  if (nan_or_inf(v) || nan_or_inf(p[0])) {
+    p[0] += v;
+  } else {
+    deposit(fold, dmindex0(p[0]), v, p);
+  }
+  renorm(fold, p, c);
+}
+
+// ------------------------------------------------------------------ kernel --
+template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK>
+__global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
+  constexpr int BM = 16 * TM, BN = 16 * TN;
+  constexpr int PAD = 2;  // keeps rows 16-byte aligned, spreads staging writes over banks
+  __shared__ __attribute__((aligned(16))) double As[KC][BM + PAD];
+  __shared__ __attribute__((aligned(16))) double Bs[KC][BN + PAD];
+  const int fold = FIXED ? CAP : g.fold;
+  const int tid = threadIdx.x;
+  const int tr = tid / 16, tc = tid % 16;
+  const int64_t i0 = (int64_t)blockIdx.y * BM, j0 = (int64_t)blockIdx.x * BN;
+
+  double p[TM][TN][CAP];
+  int c[TM][TN][CAP];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int f = 0; f < CAP; ++f) {
+        p[i][j][f] = bin(kMaxIndex + f);
+        c[i][j][f] = 0;
+      }
+  double lmin = admit_limit(kMaxIndex);  // smallest admissible limit over the lane's outputs
+  bool anytop = false;                   // some output of the lane at index 0
+  int dep = 0;
+  const double aalpha = fabs(g.alpha);
+  const int64_t chunks = (g.K + KC - 1) / KC;
+
+  for (int64_t ch = 0; ch < chunks; ++ch) {
+    const int64_t k0 = ch * KC;
+    // ---- stage A(i0.., k0..) * alpha and B(k0.., j0..) into LDS
+#pragma unroll
+    for (int r = 0; r < (BM * KC) / kThreads; ++r) {
+      int li, lk;
+      if (AI) {
+        li = tid % BM;
+        lk = tid / BM + (kThreads / BM) * r;
+      } else {
+        lk = tid % KC;
+        li = tid / KC + (kThreads / KC) * r;
+      }
+      const int64_t gi = i0 + li, gk = k0 + lk;
+      double v = (gi < g.M && gk < g.K) ? g.A[gi * g.sai + gk * g.sak] : 0.0;
+      if (g.alpha != 1.0) v = v * g.alpha;
+      As[lk][li] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < (BN * KC) / kThreads; ++r) {
+      int lj, lk;
+      if (BK) {
+        lk = tid % KC;
+        lj = tid / KC + (kThreads / KC) * r;
+      } else {
+        lj = tid % BN;
+        lk = tid / BN + (kThreads / BN) * r;
+      }
+      const int64_t gj = j0 + lj, gk = k0 + lk;
+      Bs[lk][lj] = (gj < g.N && gk < g.K) ? g.B[gk * g.sbk + gj * g.sbj] : 0.0;
+    }
+    __syncthreads();
+
+    // ---- index guard for this chunk
+    double ra = 0.0, cb = 0.0;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int64_t gi = i0 + tr * TM + i;
+      if (gi < g.M) ra = fmax(ra, g.rowmax[ch * g.M + gi]);
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int64_t gj = j0 + tc * TN + j;
+      if (gj < g.N) cb = fmax(cb, g.colmax[ch * g.N + gj]);
+    }
+    const double bound = (aalpha * ra) * cb;
+    if (!(bound < lmin)) {
+      // exact max |fl(a*b)| of each output over the chunk, then raise
+      double mx[TM][TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mx[i][j] = 0.0;
+      for (int k = 0; k < KC; ++k) {
+        double a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = As[k][tr * TM + i];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = Bs[k][tc * TN + j];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) mx[i][j] = fmax(mx[i][j], fabs(a[i] * b[j]));
+      }
+      lmin = __builtin_inf();
+      anytop = false;
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const int cur = dmindex(p[i][j][0]);
+          const int nI = dindex(mx[i][j]);
+          if (!nan_or_inf(p[i][j][0]) && nI < cur) raise_ic(fold, cur - nI, nI, p[i][j], c[i][j]);
+          const int now = dmindex(p[i][j][0]);
+          if (!nan_or_inf(p[i][j][0])) lmin = fmin(lmin, admit_limit(now));
+          anytop |= dmindex0(p[i][j][0]);
+        }
+    }
+    if (dep + KC > kEndurance) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) renorm_ic(fold, p[i][j], c[i][j]);
+      dep = 0;
+    }
+
+    // ---- deposits: 1 mul + (4F-2) add/or per product
+    if (!anytop) {
+#pragma unroll 2
+      for (int k = 0; k < KC; ++k) {
+        double a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = As[k][tr * TM + i];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = Bs[k][tc * TN + j];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) deposit(fold, false, a[i] * b[j], p[i][j]);
+      }
+    } else {
+      for (int k = 0; k < KC; ++k) {
+        double a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = As[k][tr * TM + i];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = Bs[k][tc * TN + j];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) deposit(fold, dmindex0(p[i][j][0]), a[i] * b[j], p[i][j]);
+      }
+    }
+    dep += KC;
+    __syncthreads();
+  }
+
+  // ---- epilogue
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int64_t gi = i0 + tr * TM + i, gj = j0 + tc * TN + j;
+      if (gi >= g.M || gj >= g.N) continue;
+      renorm_ic(fold, p[i][j], c[i][j]);
+      double ps[CAP], cs[CAP];
+#pragma unroll
+      for (int f = 0; f < CAP; ++f) {
+        ps[f] = p[i][j][f];
+        cs[f] = (double)c[i][j][f];
+      }
+      if (nan_or_inf(ps[0])) {
+        bool nan = false, pinf = false, ninf = false;
+        for (int64_t k = 0; k < g.K; ++k) {
+          double av = g.A[gi * g.sai + k * g.sak];
+          if (g.alpha != 1.0) av = av * g.alpha;
+          const double t = av * g.B[k * g.sbk + gj * g.sbj];
+          nan |= (t != t);
+          pinf |= (t == __builtin_inf());
+          ninf |= (t == -__builtin_inf());
+        }
+        ps[0] = (nan || (pinf && ninf) || !(pinf || ninf)) ? __builtin_nan("")
+                                                            : (pinf ? __builtin_inf() : -__builtin_inf());
+      }
+      if (g.Cb) {
+        double* cbp = g.Cb + (gi * g.cbi + gj * g.cbj) * 2 * fold;
+        double py[CAP], cy[CAP];
+#pragma unroll
+        for (int f = 0; f < CAP; ++f)
+          if (f < fold) { py[f] = cbp[f]; cy[f] = cbp[fold + f]; }
+        renorm(fold, py, cy);
+        add(fold, py, cy, ps, cs);
+        renorm(fold, py, cy);
+#pragma unroll
+        for (int f = 0; f < CAP; ++f)
+          if (f < fold) { cbp[f] = py[f]; cbp[fold + f] = cy[f]; }
+      } else {
+        double* cp = g.C + gi * g.sci + gj * g.scj;
+        double py[CAP], cy[CAP];
+        beta_init(fold, g.beta, g.beta == 0.0 ? 0.0 : *cp, py, cy);
+        add(fold, py, cy, ps, cs);
+        renorm(fold, py, cy);
+        *cp = to_double(fold, py, cy);
+      }
+    }
+  }
+}
+
+template <int CAP, bool FIXED, int TM, int TN>
+bool launch_tm(bool ai, bool bk, const GemmArgs& g, hipStream_t s) {
+  dim3 grid((unsigned)((g.N + 16 * TN - 1) / (16 * TN)), (unsigned)((g.M + 16 * TM - 1) / (16 * TM)));
+  if (ai && bk) gemm_kernel<CAP, FIXED, TM, TN, true, true><<<grid, kThreads, 0, s>>>(g);
+  else if (ai) gemm_kernel<CAP, FIXED, TM, TN, true, false><<<grid, kThreads, 0, s>>>(g);
+  else if (bk) gemm_kernel<CAP, FIXED, TM, TN, false, true><<<grid, kThreads, 0, s>>>(g);
+  else gemm_kernel<CAP, FIXED, TM, TN, false, false><<<grid, kThreads, 0, s>>>(g);
+  return RB_HIP(hipGetLastError());
+}
+
+int g_tiling = 0;  // 0: 4x4 outputs per lane, 1: 4x2, 2: 2x2
+
+template <int CAP, bool FIXED>
+bool launch_fold(bool ai, bool bk, const GemmArgs& g, hipStream_t s) {
+  const int t = FIXED ? g_tiling : 2;
+  switch (t) {
+    case 0: return launch_tm<CAP, FIXED, 4, 4>(ai, bk, g, s);
+    case 1: return launch_tm<CAP, FIXED, 4, 2>(ai, bk, g, s);
+    default: return launch_tm<CAP, FIXED, 2, 2>(ai, bk, g, s);
+  }
+}
+
+bool gemm_common(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc, double beta,
+                 double* Cb, int64_t ldcb, hipStream_t s) {
+  if (!fold_supported(fold)) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    return false;
+  }
+  const bool rm = (order == 'r' || order == 'R');
+  const bool na = (transa == 'n' || transa == 'N');
+  const bool nb = (transb == 'n' || transb == 'N');
+  GemmArgs g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.B = B;
+  // op(A)(i,k): column-major N -> A[i + k*lda]; T -> A[k + i*lda]; row-major swaps
+  if (na != rm) { g.sai = 1; g.sak = lda; } else { g.sai = lda; g.sak = 1; }
+  if (nb != rm) { g.sbk = 1; g.sbj = ldb; } else { g.sbk = ldb; g.sbj = 1; }
+  if (rm) { g.sci = ldc; g.scj = 1; g.cbi = ldcb; g.cbj = 1; } else { g.sci = 1; g.scj = ldc; g.cbi = 1; g.cbj = ldcb; }
+  g.alpha = alpha;
+  g.C = C;
+  g.beta = beta;
+  g.Cb = Cb;
+  g.fold = fold;
+  const int64_t chunks = (K + KC - 1) / KC;
+  double* mx = (double*)workspace(s, kSlotStageZ, sizeof(double) * (size_t)chunks * (M + N));
+  if (!mx) return false;
+  g.rowmax = mx;
+  g.colmax = mx + chunks * M;
+  chunk_max_kernel<<<(unsigned)((M * chunks + 255) / 256), 256, 0, s>>>(M, K, A, g.sai, g.sak, mx);
+  chunk_max_kernel<<<(unsigned)((N * chunks + 255) / 256), 256, 0, s>>>(N, K, B, g.sbj, g.sbk, mx + chunks * M);
+  if (!RB_HIP(hipGetLastError())) return false;
+  const bool ai = (g.sai == 1), bk = (g.sbk == 1);
+  switch (fold) {
+    case 2: return launch_fold<2, true>(ai, bk, g, s);
+    case 3: return launch_fold<3, true>(ai, bk, g, s);
+    case 4: return launch_fold<4, true>(ai, bk, g, s);
+    default: return launch_fold<8, false>(ai, bk, g, s);
+  }
+}
+
+}  // namespace
+
+int gemm_tiling_count() { return 3; }
+bool gemm_set_tiling(int t) {
+  if (t < 0 || t >= 3) {
+    set_error(kBadArgument, "gemm tiling %d does not exist (0..2)", t);
+    return false;
+  }
+  g_tiling = t;
+  return true;
+}
+int gemm_get_tiling() { return g_tiling; }
+
+bool dbdgemm(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double* Cb, int64_t ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0 || alpha == 0.0) return true;  // dbdgemm.ccog:77-80
+  return gemm_common(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, nullptr, 0, 0.0, Cb, ldc, s);
+}
+
+bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
+                   const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                   hipStream_t s) {
+  if (M <= 0 || N <= 0) return true;  // rdgemm.c:50-52
+  if (K <= 0 || alpha == 0.0) {
+    // no products: C still goes through binned(beta*C) and back
+    const bool rm = (order == 'r' || order == 'R');
+    const int64_t outer = rm ? M : N, inner = rm ? N : M;
+    double* tmp = (double*)workspace(s, kSlotBinnedOut, sizeof(double) * (size_t)inner * 2 * fold);
+    if (!tmp) return false;
+    for (int64_t o = 0; o < outer; ++o) {
+      if (!conv_to_binned(fold, inner, C + o * ldc, 1, beta, tmp, s)) return false;
+      if (!conv_to_double(fold, inner, tmp, C + o * ldc, 1, s)) return false;
+    }
+    return true;
+  }
+  return gemm_common(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, C, ldc, beta, nullptr, 0, s);
+}
+
+}  // namespace rb

@@ -1,0 +1,411 @@
+// gemv.hip -- binned gemv on device operands.
+//
+// Reference: binnedBLAS_dbdgemv (src/binnedBLAS/dbdgemv.ccog:57-162) under
+// reproBLAS_rdgemv (src/reproBLAS/rdgemv.c:35-87):
+//   y_i = ddbconv( dbdconv(beta*y_i) (+) sum_j fl(op(A)_ij * fl(alpha*x_j)) )
+// with alpha applied to x (dbdgemv.ccog:86-95, 110-119), beta == 0 ignoring y
+// and beta == 1 converting y unscaled (rdgemv.c:50-61). Each output is one
+// binned dot, so any split of its terms gives the same bits.
+//
+// Two memory patterns cover the four Order x Trans cases:
+//   row-dot  -- the terms of an output are contiguous (row-major N,
+//               column-major T): one wave per output streams its row with a
+//               wave-common index, like the BLAS1 dot.
+//   col-axpy -- the terms of an output are lda apart (column-major N,
+//               row-major T): each lane owns two adjacent outputs and walks
+//               the columns, so a wave reads 1 KiB contiguous per column with
+//               16-byte loads; every output keeps its own index. Column ranges
+//               are split across workgroups; a second kernel merges the
+//               per-range binned partials, folds in beta*y and converts.
+#include "binned_core.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace rb {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ double uniform_double(double v) {
+  const uint64_t u = bits(v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return from_bits(((uint64_t)hi << 32) | lo);
+}
+
+struct GemvArgs {
+  int64_t nout, len;
+  const double* A;
+  int64_t lda;
+  const double* x;
+  int64_t incx;
+  double alpha;
+  double* Yb;  // binned outputs (accumulated into) or null
+  int64_t incYb;
+  double* y;  // double outputs when Yb is null: y = conv(binned(beta*y) (+) sum)
+  int64_t incy;
+  double beta;
+  double* partials;  // col-axpy: [chunks][nout][2*fold]
+  int64_t chunk;     // col-axpy: columns per chunk
+  int64_t chunks;
+  int fold;
+};
+
+__device__ __forceinline__ double scaled_x(const GemvArgs& a, int64_t k) {
+  const double v = a.x[k * a.incx];
+  return a.alpha == 1.0 ? v : v * a.alpha;
+}
+
+// binned(beta*y) as rdgemv.c:50-61 forms it (binned_dbdconv = dmdadd on zero)
+template <int CAP>
+__device__ __forceinline__ void beta_init(int fold, double beta, double yv, double (&p)[CAP], double (&c)[CAP]) {
+  set_zero(fold, p, c);
+  if (beta == 0.0) return;
+  const double v = (beta == 1.0) ? yv : yv * beta;
+  update(fold, v, p, c);
+  if (nan_or_inf(v) || nan_or_inf(p[0])) {
+    p[0] += v;
+  } else {
+    deposit(fold, dmindex0(p[0]), v, p);
+  }
+  renorm(fold, p, c);
+}
+
+// Final step for one output: fold the sum S into the caller's value.
+template <int CAP>
+__device__ __forceinline__ void finish_output(const GemvArgs& a, int fold, int64_t o, double (&p)[CAP],
+                                              double (&c)[CAP]) {
+  if (a.Yb) {
+    double* yb = a.Yb + o * a.incYb * 2 * fold;
+    double py[CAP], cy[CAP];
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) { py[j] = yb[j]; cy[j] = yb[fold + j]; }
+    renorm(fold, py, cy);
+    add(fold, py, cy, p, c);
+    renorm(fold, py, cy);
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) { yb[j] = py[j]; yb[fold + j] = cy[j]; }
+  } else {
+    double py[CAP], cy[CAP];
+    beta_init(fold, a.beta, a.beta == 0.0 ? 0.0 : a.y[o * a.incy], py, cy);
+    add(fold, py, cy, p, c);
+    renorm(fold, py, cy);
+    a.y[o * a.incy] = to_double(fold, py, cy);
+  }
+}
+
+// ---------------------------------------------------------------- row-dot --
+template <int CAP, bool FIXED>
+__global__ __launch_bounds__(kThreads) void rowdot_kernel(GemvArgs a) {
+  constexpr int U = 8;
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int64_t o = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (o >= a.nout) return;  // whole wave leaves together
+  const double* row = a.A + o * a.lda;
+  double p[CAP], c[CAP];
+  int idx = kMaxIndex;
+  set_bins(fold, idx, p, c);
+  double lim = admit_limit(idx);
+  int dep = 0;
+  const int64_t tiles = (a.len + 64 * U - 1) / (64 * U);
+  for (int64_t t = 0; t < tiles; ++t) {
+    double v[U];
+    double m = 0.0;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t e = t * (64 * U) + k * 64 + lane;
+      v[k] = (e < a.len) ? row[e] * scaled_x(a, e) : 0.0;
+      m = fmax(m, fabs(v[k]));
+    }
+    if (dep + U > kEndurance) {
+      renorm(fold, p, c);
+      dep = 0;
+    }
+    if (__any(m >= lim)) {
+      double M = m;
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1) M = fmax(M, shfl_xor_d(M, s));
+      const int nI = dindex(uniform_double(M));
+      if (nI < idx) {
+        raise_index(fold, idx - nI, nI, p, c);
+        idx = nI;
+      }
+      lim = admit_limit(idx);
+    }
+    const bool top = (idx == 0);
+#pragma unroll
+    for (int k = 0; k < U; ++k) deposit(fold, top, v[k], p);
+    dep += U;
+  }
+  renorm(fold, p, c);
+#pragma unroll
+  for (int s = 1; s <= 32; s <<= 1) {
+    double op[CAP], oc[CAP];
+#pragma unroll
+    for (int j = 0; j < CAP; ++j) {
+      op[j] = shfl_xor_d(p[j], s);
+      oc[j] = shfl_xor_d(c[j], s);
+    }
+    if (!nan_or_inf(p[0]) && !nan_or_inf(op[0])) {
+      add_same_index(fold, idx, p, c, op, oc);
+    } else {
+      p[0] = p[0] + op[0];
+    }
+  }
+  if (nan_or_inf(uniform_double(p[0]))) {
+    // class of the terms, as the reference's infinity path would add them
+    bool nan = false, pinf = false, ninf = false;
+    for (int64_t e = lane; e < a.len; e += 64) {
+      const double t = row[e] * scaled_x(a, e);
+      nan |= (t != t);
+      pinf |= (t == __builtin_inf());
+      ninf |= (t == -__builtin_inf());
+    }
+    const bool an = __any(nan), ap = __any(pinf), aneg = __any(ninf);
+    p[0] = (an || (ap && aneg) || !(ap || aneg)) ? __builtin_nan("") : (ap ? __builtin_inf() : -__builtin_inf());
+  }
+  if (lane == 0) finish_output(a, fold, o, p, c);
+}
+
+// --------------------------------------------------------------- col-axpy --
+// Block: 128 outputs (2 per lane) x one column chunk; wave w takes columns
+// chunk_start + w, +4, +8, ...
+template <int CAP, bool FIXED, bool VEC>
+__global__ __launch_bounds__(kThreads) void colaxpy_kernel(GemvArgs a) {
+  constexpr int U = 4;  // columns per step
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t o0 = (int64_t)blockIdx.x * 128 + 2 * lane;
+  const int64_t kbeg = (int64_t)blockIdx.y * a.chunk;
+  int64_t kend = kbeg + a.chunk;
+  if (kend > a.len) kend = a.len;
+  const bool in0 = o0 < a.nout, in1 = o0 + 1 < a.nout;
+
+  double p0[CAP], c0[CAP], p1[CAP], c1[CAP];
+  int idx0 = kMaxIndex, idx1 = kMaxIndex;
+  set_bins(fold, kMaxIndex, p0, c0);
+  set_bins(fold, kMaxIndex, p1, c1);
+  double lim0 = admit_limit(kMaxIndex), lim1 = lim0;
+  int dep = 0;
+  for (int64_t k = kbeg + wave; k < kend; k += kWaves * U) {
+    double t0[U], t1[U];
+    double m0 = 0.0, m1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t kk = k + (int64_t)u * kWaves;
+      const bool kin = kk < kend;
+      double a0 = 0.0, a1 = 0.0, xs = 0.0;
+      if (kin) {
+        xs = scaled_x(a, kk);
+        const double* col = a.A + kk * a.lda + o0;
+        if (VEC && in1) {
+          const double2 d = *reinterpret_cast<const double2*>(col);
+          a0 = d.x;
+          a1 = d.y;
+        } else {
+          a0 = in0 ? col[0] : 0.0;
+          a1 = in1 ? col[1] : 0.0;
+        }
+      }
+      t0[u] = a0 * xs;
+      t1[u] = a1 * xs;
+      m0 = fmax(m0, fabs(t0[u]));
+      m1 = fmax(m1, fabs(t1[u]));
+    }
+    if (dep + U > kEndurance) {
+      renorm(fold, p0, c0);
+      renorm(fold, p1, c1);
+      dep = 0;
+    }
+    if (__any(m0 >= lim0 || m1 >= lim1)) {
+      // every output has its own index: per-lane raise
+      const int n0 = dindex(m0), n1 = dindex(m1);
+      if (n0 < idx0) {
+        raise_index(fold, idx0 - n0, n0, p0, c0);
+        idx0 = n0;
+        lim0 = admit_limit(idx0);
+      }
+      if (n1 < idx1) {
+        raise_index(fold, idx1 - n1, n1, p1, c1);
+        idx1 = n1;
+        lim1 = admit_limit(idx1);
+      }
+    }
+    const bool top0 = idx0 == 0, top1 = idx1 == 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      deposit(fold, top0, t0[u], p0);
+      deposit(fold, top1, t1[u], p1);
+    }
+    dep += U;
+  }
+  renorm(fold, p0, c0);
+  renorm(fold, p1, c1);
+
+  // merge the four waves' values per output (general add), wave 0 writes
+  __shared__ double lds[kWaves - 1][128][2 * CAP];
+  if (wave > 0) {
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        lds[wave - 1][2 * lane][j] = p0[j];
+        lds[wave - 1][2 * lane][CAP + j] = c0[j];
+        lds[wave - 1][2 * lane + 1][j] = p1[j];
+        lds[wave - 1][2 * lane + 1][CAP + j] = c1[j];
+      }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  for (int w = 0; w < kWaves - 1; ++w) {
+    double qp[CAP], qc[CAP];
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) { qp[j] = lds[w][2 * lane][j]; qc[j] = lds[w][2 * lane][CAP + j]; }
+    add(fold, p0, c0, qp, qc);
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) { qp[j] = lds[w][2 * lane + 1][j]; qc[j] = lds[w][2 * lane + 1][CAP + j]; }
+    add(fold, p1, c1, qp, qc);
+  }
+  double* part = a.partials + ((int64_t)blockIdx.y * a.nout) * 2 * fold;
+  if (in0) {
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) { part[o0 * 2 * fold + j] = p0[j]; part[o0 * 2 * fold + fold + j] = c0[j]; }
+  }
+  if (in1) {
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) { part[(o0 + 1) * 2 * fold + j] = p1[j]; part[(o0 + 1) * 2 * fold + fold + j] = c1[j]; }
+  }
+}
+
+// merge the per-chunk partials of each output, fold in beta*y (or Yb), convert
+template <int CAP, bool FIXED>
+__global__ __launch_bounds__(kThreads) void colaxpy_finish(GemvArgs a) {
+  const int fold = FIXED ? CAP : a.fold;
+  const int64_t o = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (o >= a.nout) return;
+  double p[CAP], c[CAP];
+  const double* src = a.partials + o * 2 * fold;
+#pragma unroll
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) { p[j] = src[j]; c[j] = src[fold + j]; }
+  for (int64_t s = 1; s < a.chunks; ++s) {
+    const double* q = a.partials + (s * a.nout + o) * 2 * fold;
+    double qp[CAP], qc[CAP];
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) { qp[j] = q[j]; qc[j] = q[fold + j]; }
+    add(fold, p, c, qp, qc);
+  }
+  if (nan_or_inf(p[0])) {
+    bool nan = false, pinf = false, ninf = false;
+    for (int64_t k = 0; k < a.len; ++k) {
+      const double t = a.A[o + k * a.lda] * scaled_x(a, k);
+      nan |= (t != t);
+      pinf |= (t == __builtin_inf());
+      ninf |= (t == -__builtin_inf());
+    }
+    p[0] = (nan || (pinf && ninf) || !(pinf || ninf)) ? __builtin_nan("") : (pinf ? __builtin_inf() : -__builtin_inf());
+  }
+  finish_output(a, fold, o, p, c);
+}
+
+template <int CAP, bool FIXED>
+bool launch_gemv(bool rowdot, GemvArgs& a, hipStream_t s) {
+  if (rowdot) {
+    const int64_t blocks = (a.nout + kWaves - 1) / kWaves;
+    rowdot_kernel<CAP, FIXED><<<(unsigned)blocks, kThreads, 0, s>>>(a);
+    return RB_HIP(hipGetLastError());
+  }
+  const DeviceInfo* d = device_info();
+  if (!d) return false;
+  const int64_t rowblocks = (a.nout + 127) / 128;
+  int64_t chunks = (4 * (int64_t)d->cus + rowblocks - 1) / rowblocks;
+  const int64_t max_chunks = (a.len + 255) / 256;  // keep >= 256 columns per chunk
+  if (chunks > max_chunks) chunks = max_chunks;
+  if (chunks < 1) chunks = 1;
+  if (chunks > 65535) chunks = 65535;
+  a.chunk = (a.len + chunks - 1) / chunks;
+  a.chunks = (a.len + a.chunk - 1) / a.chunk;
+  a.partials = (double*)workspace(s, kSlotPartials, sizeof(double) * (size_t)a.chunks * a.nout * 2 * a.fold);
+  if (!a.partials) return false;
+  const bool vec = ((uintptr_t)a.A % 16 == 0) && (a.lda % 2 == 0);
+  dim3 grid((unsigned)rowblocks, (unsigned)a.chunks);
+  if (vec)
+    colaxpy_kernel<CAP, FIXED, true><<<grid, kThreads, 0, s>>>(a);
+  else
+    colaxpy_kernel<CAP, FIXED, false><<<grid, kThreads, 0, s>>>(a);
+  if (!RB_HIP(hipGetLastError())) return false;
+  colaxpy_finish<CAP, FIXED><<<(unsigned)((a.nout + kThreads - 1) / kThreads), kThreads, 0, s>>>(a);
+  return RB_HIP(hipGetLastError());
+}
+
+bool gemv_common(int fold, char order, char trans, int64_t M, int64_t N, double alpha, const double* A, int64_t lda,
+                 const double* x, int64_t incx, double* Yb, int64_t incYb, double* y, int64_t incy, double beta,
+                 hipStream_t s) {
+  if (!fold_supported(fold)) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    return false;
+  }
+  const bool rowmajor = (order == 'r' || order == 'R');
+  const bool notrans = (trans == 'n' || trans == 'N');
+  GemvArgs a;
+  a.nout = notrans ? M : N;
+  a.len = notrans ? N : M;
+  a.A = A;
+  a.lda = lda;
+  a.x = x;
+  a.incx = incx;
+  a.alpha = alpha;
+  a.Yb = Yb;
+  a.incYb = incYb;
+  a.y = y;
+  a.incy = incy;
+  a.beta = beta;
+  a.partials = nullptr;
+  a.chunk = a.chunks = 0;
+  a.fold = fold;
+  if (a.nout <= 0) return true;
+  // the terms of an output are contiguous for row-major N and column-major T
+  const bool rowdot = (rowmajor == notrans);
+  switch (fold) {
+    case 2: return launch_gemv<2, true>(rowdot, a, s);
+    case 3: return launch_gemv<3, true>(rowdot, a, s);
+    case 4: return launch_gemv<4, true>(rowdot, a, s);
+    default: return launch_gemv<8, false>(rowdot, a, s);
+  }
+}
+
+}  // namespace
+
+bool dbdgemv(int fold, char order, char trans, int64_t M, int64_t N, double alpha, const double* A, int64_t lda,
+             const double* x, int64_t incx, double* Yb, int64_t incYb, hipStream_t s) {
+  if (M <= 0 || N <= 0 || alpha == 0.0) return true;  // dbdgemv.ccog:75-77
+  return gemv_common(fold, order, trans, M, N, alpha, A, lda, x, incx, Yb, incYb, nullptr, 0, 0.0, s);
+}
+
+// y = conv(binned(beta*y) (+) sum) in place, for reproBLAS_rdgemv
+bool rdgemv_device(int fold, char order, char trans, int64_t M, int64_t N, double alpha, const double* A,
+                   int64_t lda, const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s) {
+  if (M <= 0 || N <= 0) return true;  // rdgemv.c:43-45
+  const bool notrans = (trans == 'n' || trans == 'N');
+  const int64_t nout = notrans ? M : N;
+  if (alpha == 0.0) {
+    // dbdgemv adds nothing; y still goes through binned(beta*y) and back
+    double* tmp = (double*)workspace(s, kSlotBinnedOut, sizeof(double) * (size_t)nout * 2 * fold);
+    if (!tmp) return false;
+    return conv_to_binned(fold, nout, y, incy, beta, tmp, s) && conv_to_double(fold, nout, tmp, y, incy, s);
+  }
+  return gemv_common(fold, order, trans, M, N, alpha, A, lda, x, incx, nullptr, 0, y, incy, beta, s);
+}
+
+}  // namespace rb

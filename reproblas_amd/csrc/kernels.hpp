@@ -1,0 +1,63 @@
+// kernels.hpp -- launchers of the HIP kernels (host-callable, device operands).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rb {
+
+// Folds with a dedicated (fully register-resident) kernel instance; every
+// other fold in [2, kMaxFold] runs the general instance.
+bool fold_supported(int fold);
+
+enum Term : int { kSum = 0, kASum = 1, kDot = 2, kSSq = 3 };
+
+// One reduction of the binned BLAS1 family over device vectors:
+//   kSum: t_i = x_i, kASum: |x_i|, kDot: fl(x_i*y_i), kSSq: fl(fl(x_i/scl)^2)
+// state_in  -- [2*fold] binned value to accumulate into, or null for zero
+// scale_in  -- kSSq only: the scale of state_in (0 when none)
+// scale_in_dev -- kSSq only: device copy of that scale; used instead when set
+// state_out -- [2*fold] final binned value, or null
+// scale_out -- kSSq only: the final scale, or null
+// result    -- converted double (kSSq: scl*sqrt(conv)), or null
+// All device pointers; runs on stream s without synchronising.
+bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
+           const double* state_in, double scale_in, const double* scale_in_dev, double* state_out,
+           double* scale_out, double* result, hipStream_t s);
+
+// Elementwise binned helpers on device arrays of binned values.
+// dbdbaddv: Y[i] += X[i] for i < n (stride incX/incY in binned elements).
+bool dbdbaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int64_t incY, hipStream_t s);
+// conv_to_binned: Y[i] = dbdconv(beta == 1 ? y[i] : y[i]*beta) or zero when beta == 0.
+bool conv_to_binned(int fold, int64_t n, const double* y, int64_t incy, double beta, double* Yb, hipStream_t s);
+// conv_to_double: y[i] = ddbconv(Yb[i]).
+bool conv_to_double(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s);
+
+// Binned gemv on device operands (reference binnedBLAS_dbdgemv semantics,
+// column- or row-major): Yb[i*incY] (+)= sum_j fl(op(A)_ij * fl(alpha*x_j)).
+bool dbdgemv(int fold, char order, char trans, int64_t M, int64_t N, double alpha, const double* A, int64_t lda,
+             const double* x, int64_t incx, double* Yb, int64_t incYb, hipStream_t s);
+
+// Binned gemm on device operands (reference binnedBLAS_dbdgemm semantics):
+// C(i,j) (+)= sum_k fl(fl(alpha*op(A)_ik) * op(B)_kj), C binned with
+// leading dimension ldc (in binned elements).
+bool dbdgemm(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double* Cb, int64_t ldc, hipStream_t s);
+
+// In-place y = conv(binned(beta*y) (+) sum_j fl(op(A)_ij * fl(alpha*x_j)))
+// (reproBLAS_rdgemv) on device operands.
+bool rdgemv_device(int fold, char order, char trans, int64_t M, int64_t N, double alpha, const double* A,
+                   int64_t lda, const double* x, int64_t incx, double beta, double* y, int64_t incy, hipStream_t s);
+
+// In-place C = conv(binned(beta*C) (+) sum_k fl(fl(alpha*op(A)_ik) * op(B)_kj))
+// (reproBLAS_rdgemm) on device operands.
+bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
+                   const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                   hipStream_t s);
+
+// gemm tilings (outputs per lane 4x4, 4x2, 2x2); all give identical bits
+int gemm_tiling_count();
+bool gemm_set_tiling(int t);
+int gemm_get_tiling();
+
+}  // namespace rb

@@ -1,0 +1,69 @@
+// runtime.hpp -- host-side runtime of the HIP library: error state, the
+// library stream, device properties, cached device workspaces and staging of
+// host operands.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace rb {
+
+// Error codes reported through reproblas_hip_last_error().
+enum Status : int {
+  kOk = 0,
+  kBadFold = 1,      // fold outside the range the library supports
+  kBadArgument = 2,  // negative size, bad leading dimension, null pointer
+  kHipError = 3,     // a HIP runtime call failed
+  kNoDevice = 4,     // no usable GPU
+  kUnsupported = 5,  // a case this build does not handle
+};
+
+void set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+int last_error();
+const char* last_error_message();
+
+// Checks a HIP call; on failure records kHipError with the call site and
+// returns false.
+bool hip_ok(hipError_t e, const char* what);
+#define RB_HIP(call) ::rb::hip_ok((call), #call)
+
+// Stream used by the synchronous entry points (default: the null stream).
+hipStream_t current_stream();
+void set_current_stream(hipStream_t s);
+
+struct DeviceInfo {
+  int device = -1;
+  int cus = 0;
+  int warp = 64;
+  size_t lds_per_block = 0;
+};
+// Properties of the current device (cached).
+const DeviceInfo* device_info();
+
+// Device memory of at least `bytes` owned by the library, one slot per
+// (device, stream, slot id), grown on demand and kept until process exit.
+// Calls that share a stream are ordered by that stream, so sharing a slot
+// between them is safe.
+void* workspace(hipStream_t s, int slot, size_t bytes);
+
+// Zero-initialised device counters (one per slot) used by the single-pass
+// grid reductions; every kernel that takes one leaves it at zero.
+unsigned* tickets(hipStream_t s, int count);
+
+// Pointer kind of `p`: true when a kernel may dereference it directly.
+bool is_device_pointer(const void* p);
+
+enum WorkspaceSlot : int {
+  kSlotPartials = 0,
+  kSlotResult = 1,
+  kSlotStageX = 2,
+  kSlotStageY = 3,
+  kSlotStageZ = 4,
+  kSlotBinnedOut = 5,
+  kSlotBatch = 6,
+  kSlotCount = 8,
+};
+
+}  // namespace rb

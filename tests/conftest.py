@@ -1,0 +1,28 @@
+import os
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+for p in (HERE, ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+    config.addinivalue_line("markers", "slow: full-size (BASELINE-scale) GPU case")
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """torch on the GPU; the HIP library must load (no fallback)."""
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import reproblas_amd
+
+    reproblas_amd.lib()
+    return torch.device("cuda:0")

@@ -1,0 +1,179 @@
+"""ctypes handles on the CPU checker libraries (test infrastructure only).
+
+* ``oracle()``  -- oracle/liboracle.so, the scalar restatement (orc_* symbols).
+* ``reference(kind)`` -- oracle/_ref/libreproblas_ref{,_sisd}.so, the
+  reference's own C sources compiled by oracle/Makefile; ``None`` when they
+  were not built (the reference tree is only present where the fixtures are
+  made).
+
+Both are wrapped in :class:`CpuBlas`, which exposes the reproBLAS-style calls
+on numpy arrays so a test can run the same call on either library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = {
+    "avx": os.path.join(ROOT, "oracle", "_ref", "libreproblas_ref.so"),
+    "sisd": os.path.join(ROOT, "oracle", "_ref", "libreproblas_ref_sisd.so"),
+}
+
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+_c_char = ctypes.c_char
+_vp = ctypes.c_void_p
+
+
+def _ptr(a):
+    if not a.flags["C_CONTIGUOUS"] or a.dtype != np.float64:
+        raise ValueError("CPU checker calls need contiguous float64 arrays")
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class CpuBlas:
+    """reproBLAS calls on numpy float64 arrays through one CPU library."""
+
+    def __init__(self, path, style):
+        self.path = path
+        self.style = style  # "orc" or "ref"
+        self.lib = ctypes.CDLL(path)
+        n = self._name
+        sig = {
+            "rdsum": (_c_double, [_c_int, _c_int, _vp, _c_int]),
+            "rdasum": (_c_double, [_c_int, _c_int, _vp, _c_int]),
+            "rdnrm2": (_c_double, [_c_int, _c_int, _vp, _c_int]),
+            "rddot": (_c_double, [_c_int, _c_int, _vp, _c_int, _vp, _c_int]),
+            "rdgemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _c_double, _vp, _c_int,
+                              _vp, _c_int, _c_double, _vp, _c_int]),
+            "rdgemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _c_double,
+                              _vp, _c_int, _vp, _c_int, _c_double, _vp, _c_int]),
+            "dbdsum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
+            "dbdasum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
+            "dbddot": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp]),
+            "dbdssq": (_c_double, [_c_int, _c_int, _vp, _c_int, _c_double, _vp]),
+            "dbdgemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _c_double, _vp, _c_int,
+                               _vp, _c_int, _vp, _c_int]),
+            "dbdgemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _c_double,
+                               _vp, _c_int, _vp, _c_int, _vp, _c_int]),
+            "dbdbadd": (None, [_c_int, _vp, _vp]),
+            "dbdbaddsq": (_c_double, [_c_int, _c_double, _vp, _c_double, _vp]),
+            "dbdconv": (None, [_c_int, _c_double, _vp]),
+            "ddbconv": (_c_double, [_c_int, _vp]),
+            "dbdadd": (None, [_c_int, _c_double, _vp]),
+            "dindex": (_c_int, [_c_double]),
+            "dmindex": (_c_int, [_vp]),
+            "dscale": (_c_double, [_c_double]),
+            "dbbound": (_c_double, [_c_int, _c_int, _c_double, _c_double]),
+            "damax": (_c_double, [_c_int, _vp, _c_int]),
+            "damaxm": (_c_double, [_c_int, _vp, _c_int, _vp, _c_int]),
+        }
+        self.fn = {}
+        for k, (res, args) in sig.items():
+            f = getattr(self.lib, n(k))
+            f.restype = res
+            f.argtypes = args
+            self.fn[k] = f
+
+    def _name(self, k):
+        if self.style == "orc":
+            return "orc_" + k
+        if k in ("rdsum", "rdasum", "rdnrm2", "rddot", "rdgemv", "rdgemm"):
+            return "reproBLAS_" + k
+        if k in ("dbdsum", "dbdasum", "dbddot", "dbdssq", "dbdgemv", "dbdgemm", "damax", "damaxm"):
+            return "binnedBLAS_" + k
+        return "binned_" + k
+
+    # --- BLAS1 -----------------------------------------------------------
+    def rdsum(self, fold, x, inc=1, n=None):
+        n = len(x) // max(abs(inc), 1) if n is None else n
+        return self.fn["rdsum"](fold, n, _ptr(x), inc)
+
+    def rdasum(self, fold, x, inc=1, n=None):
+        n = len(x) // max(abs(inc), 1) if n is None else n
+        return self.fn["rdasum"](fold, n, _ptr(x), inc)
+
+    def rdnrm2(self, fold, x, inc=1, n=None):
+        n = len(x) // max(abs(inc), 1) if n is None else n
+        return self.fn["rdnrm2"](fold, n, _ptr(x), inc)
+
+    def rddot(self, fold, x, y, incx=1, incy=1, n=None):
+        n = min(len(x) // max(abs(incx), 1), len(y) // max(abs(incy), 1)) if n is None else n
+        return self.fn["rddot"](fold, n, _ptr(x), incx, _ptr(y), incy)
+
+    # --- binned-state BLAS1 (accumulate into Y, a float64 array of 2*fold) --
+    def dbdsum(self, fold, x, Y, inc=1):
+        self.fn["dbdsum"](fold, len(x) // abs(inc), _ptr(x), inc, _ptr(Y))
+
+    def dbdasum(self, fold, x, Y, inc=1):
+        self.fn["dbdasum"](fold, len(x) // abs(inc), _ptr(x), inc, _ptr(Y))
+
+    def dbddot(self, fold, x, y, Z):
+        self.fn["dbddot"](fold, len(x), _ptr(x), 1, _ptr(y), 1, _ptr(Z))
+
+    def dbdssq(self, fold, x, scale, Y):
+        return self.fn["dbdssq"](fold, len(x), _ptr(x), 1, scale, _ptr(Y))
+
+    # --- primitives ---------------------------------------------------------
+    def dbdbadd(self, fold, X, Y):
+        self.fn["dbdbadd"](fold, _ptr(X), _ptr(Y))
+
+    def dbdbaddsq(self, fold, sx, X, sy, Y):
+        return self.fn["dbdbaddsq"](fold, sx, _ptr(X), sy, _ptr(Y))
+
+    def dbdconv(self, fold, x):
+        Y = np.zeros(2 * fold)
+        self.fn["dbdconv"](fold, x, _ptr(Y))
+        return Y
+
+    def ddbconv(self, fold, Y):
+        return self.fn["ddbconv"](fold, _ptr(Y))
+
+    def dbdadd(self, fold, x, Y):
+        self.fn["dbdadd"](fold, x, _ptr(Y))
+
+    def dindex(self, x):
+        return self.fn["dindex"](x)
+
+    def dmindex(self, Y):
+        return self.fn["dmindex"](_ptr(Y))
+
+    def dscale(self, x):
+        return self.fn["dscale"](x)
+
+    def dbbound(self, fold, n, x, s):
+        return self.fn["dbbound"](fold, n, x, s)
+
+    # --- level 2/3 ----------------------------------------------------------
+    def rdgemv(self, fold, order, trans, M, N, alpha, A, lda, x, incx, beta, y, incy):
+        self.fn["rdgemv"](fold, order.encode(), trans.encode(), M, N, alpha, _ptr(A), lda, _ptr(x), incx,
+                          beta, _ptr(y), incy)
+
+    def rdgemm(self, fold, order, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
+        self.fn["rdgemm"](fold, order.encode(), ta.encode(), tb.encode(), M, N, K, alpha, _ptr(A), lda,
+                          _ptr(B), ldb, beta, _ptr(C), ldc)
+
+    def dbdgemm(self, fold, order, ta, tb, M, N, K, alpha, A, lda, B, ldb, C, ldc):
+        self.fn["dbdgemm"](fold, order.encode(), ta.encode(), tb.encode(), M, N, K, alpha, _ptr(A), lda,
+                           _ptr(B), ldb, _ptr(C), ldc)
+
+
+_cache = {}
+
+
+def oracle():
+    if "orc" not in _cache:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+        _cache["orc"] = CpuBlas(ORACLE_SO, "orc")
+    return _cache["orc"]
+
+
+def reference(kind="avx"):
+    key = "ref_" + kind
+    if key not in _cache:
+        p = REF_SO[kind]
+        _cache[key] = CpuBlas(p, "ref") if os.path.exists(p) else None
+    return _cache[key]

@@ -1,0 +1,79 @@
+"""Seeded synthetic inputs shared by the tests and the benchmark.
+
+All draws use numpy's PCG64, which is platform independent, so the same seed
+gives the same vector here and on the GPU host.
+"""
+import numpy as np
+
+
+def rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def uniform(n, seed, lo=-1.0, hi=1.0):
+    return rng(seed).uniform(lo, hi, n)
+
+
+def wide(n, seed, emin=-40, emax=40):
+    """x = s * m * 2^e, s = +-1, m ~ U[1,2), e ~ U{emin..emax}."""
+    g = rng(seed)
+    s = g.choice(np.array([-1.0, 1.0]), n)
+    m = g.uniform(1.0, 2.0, n)
+    e = g.integers(emin, emax + 1, n)
+    return s * np.ldexp(m, e)
+
+
+def ill_conditioned(n, seed, noise=1e-12):
+    """N/2 pairs (x, -x + noise*u) interleaved, then shuffled by `seed`.
+
+    The multiset of values depends only on n (it is drawn from seed 0);
+    `seed` only permutes it, so a reproducible sum must not depend on it."""
+    g = rng(0)
+    half = n // 2
+    x = g.uniform(-1.0, 1.0, half) * np.exp2(g.integers(-20, 21, half).astype(float))
+    v = np.empty(n)
+    v[0:2 * half:2] = x
+    v[1:2 * half:2] = -x + noise * g.uniform(-1.0, 1.0, half)
+    if n > 2 * half:
+        v[-1] = noise * g.uniform(-1.0, 1.0)
+    if seed == 0:
+        return v
+    return rng(seed).permutation(v)
+
+
+def normal(shape, seed, scale=1.0):
+    return rng(seed).normal(0.0, scale, shape)
+
+
+def special(kind, n, seed):
+    """Inputs with non-finite or extreme members."""
+    x = uniform(n, seed)
+    g = rng(seed + 1000)
+    if kind == "nan":
+        x[g.integers(0, n)] = np.nan
+    elif kind == "pinf":
+        x[g.integers(0, n)] = np.inf
+    elif kind == "ninf":
+        x[g.integers(0, n)] = -np.inf
+    elif kind == "both_inf":
+        i, j = g.choice(n, 2, replace=False)
+        x[i], x[j] = np.inf, -np.inf
+    elif kind == "inf_nan":
+        i, j = g.choice(n, 2, replace=False)
+        x[i], x[j] = np.inf, np.nan
+    elif kind == "big":  # index-0 (compressed top bin) path
+        x = x * np.ldexp(1.0, 1020)
+        x[g.integers(0, n)] = 1.7976931348623157e308
+    elif kind == "tiny":  # subnormals and near-underflow
+        x = x * np.ldexp(1.0, -1060)
+    elif kind == "mixed_range":
+        x = x * np.exp2(g.integers(-1000, 1000, n).astype(float))
+    elif kind == "zeros":
+        x = np.zeros(n)
+    elif kind == "neg_zeros":
+        x = -np.zeros(n)
+    elif kind == "constant":
+        x = np.full(n, 0.1)
+    else:
+        raise ValueError(kind)
+    return x

@@ -1,0 +1,164 @@
+"""GPU parity of gemv and gemm against the oracle (bit-identical outputs).
+
+Reference behaviour: src/reproBLAS/rdgemv.c:35-87 with dbdgemv.ccog
+(alpha on x) and src/reproBLAS/rdgemm.c:41-105 with dbdgemm.ccog (alpha on
+A); beta == 0 ignores the output, beta == 1 converts it unscaled.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import datagen as dg
+from cpu_libs import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def eq_arrays(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all(nan | (a.view(np.uint64) == b.view(np.uint64))))
+
+
+def vp(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+@pytest.mark.parametrize("order", ["C", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("shape", [(1, 1), (7, 5), (255, 512), (300, 77), (1000, 2049)])
+@pytest.mark.parametrize("ab", [(1.0, 0.0), (2.0, 0.5), (0.5, 1.0), (0.0, 2.0)])
+def test_rdgemv(gpu, order, trans, shape, ab):
+    import reproblas_amd as rb
+
+    M, N = shape
+    alpha, beta = ab
+    lda = (M if order == "C" else N) + 3
+    A = dg.normal(lda * (N if order == "C" else M), 61)
+    lx = N if trans == "N" else M
+    ly = M if trans == "N" else N
+    x = dg.normal(lx, 62)
+    y0 = dg.normal(ly, 63)
+    for fold in (2, 3, 4):
+        ye = y0.copy()
+        oracle().fn["rdgemv"](fold, order.encode(), trans.encode(), M, N, alpha, vp(A), lda, vp(x), 1, beta, vp(ye), 1)
+        yg = y0.copy()
+        rb.lib().reproBLAS_rdgemv(fold, order.encode(), trans.encode(), M, N, alpha, vp(A), lda, vp(x), 1, beta,
+                                  vp(yg), 1)
+        rb._check()
+        assert eq_arrays(yg, ye), (order, trans, shape, ab, fold)
+
+
+@pytest.mark.parametrize("order", ["C", "R"])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (64, 64, 64), (96, 80, 72), (33, 65, 130), (128, 40, 2100)])
+@pytest.mark.parametrize("ab", [(1.0, 0.0), (2.0, 0.5), (0.5, 1.0)])
+def test_rdgemm(gpu, order, ta, tb, mnk, ab):
+    import reproblas_amd as rb
+
+    M, N, K = mnk
+    alpha, beta = ab
+    rm = order == "R"
+    ar, ac = (M, K) if ta == "N" else (K, M)
+    br, bc = (K, N) if tb == "N" else (N, K)
+    lda = (ac if rm else ar) + 1
+    ldb = (bc if rm else br) + 2
+    ldc = (N if rm else M) + 1
+    A = dg.normal(lda * (ar if rm else ac), 71)
+    B = dg.normal(ldb * (br if rm else bc), 72)
+    C0 = dg.normal(ldc * (M if rm else N), 73)
+    for fold in (3,) if M * N * K > 100000 else (2, 3, 4):
+        Ce = C0.copy()
+        oracle().fn["rdgemm"](fold, order.encode(), ta.encode(), tb.encode(), M, N, K, alpha, vp(A), lda, vp(B), ldb,
+                              beta, vp(Ce), ldc)
+        Cg = C0.copy()
+        rb.lib().reproBLAS_rdgemm(fold, order.encode(), ta.encode(), tb.encode(), M, N, K, alpha, vp(A), lda, vp(B),
+                                  ldb, beta, vp(Cg), ldc)
+        rb._check()
+        assert eq_arrays(Cg, Ce), (order, ta, tb, mnk, ab, fold)
+
+
+@pytest.mark.parametrize("tiling", [0, 1, 2])
+def test_gemm_tilings_identical(gpu, tiling):
+    """Every tiling gives the same bits (and the oracle's)."""
+    import torch
+
+    import reproblas_amd as rb
+
+    M, N, K = 160, 96, 700
+    A = dg.wide(M * K, 81).reshape(K, M)  # column-major M x K
+    B = dg.wide(K * N, 82).reshape(N, K)
+    C0 = dg.normal((N, M), 83)
+    Ce = C0.copy()
+    oracle().fn["rdgemm"](3, b"C", b"N", b"N", M, N, K, 1.0, vp(A), M, vp(B), K, 0.5, vp(Ce), M)
+    old = rb.get_gemm_tiling()
+    try:
+        rb.set_gemm_tiling(tiling)
+        tA, tB = torch.from_numpy(A).to(gpu), torch.from_numpy(B).to(gpu)
+        tC = torch.from_numpy(C0.copy()).to(gpu)
+        rb.rdgemm("N", "N", 1.0, tA, tB, 0.5, tC)
+        assert eq_arrays(tC.cpu().numpy(), Ce)
+    finally:
+        rb.set_gemm_tiling(old)
+
+
+def test_gemm_special_values(gpu):
+    import reproblas_amd as rb
+
+    M, N, K = 40, 36, 50
+    A = dg.normal((K, M), 91)
+    B = dg.normal((N, K), 92)
+    A[3, 5] = np.inf
+    A[7, 9] = -np.inf
+    B[2, 11] = np.nan
+    A[10, 20] = 1e300
+    B[4, 10] = 1e300  # overflowing product
+    C0 = dg.normal((N, M), 93)
+    Ce = C0.copy()
+    oracle().fn["rdgemm"](3, b"C", b"N", b"N", M, N, K, 1.0, vp(A), M, vp(B), K, 0.0, vp(Ce), M)
+    Cg = C0.copy()
+    rb.rdgemm("N", "N", 1.0, A, B, 0.0, Cg)
+    assert eq_arrays(Cg, Ce)
+
+
+def test_gemm_binned_accumulate(gpu):
+    """binnedBLAS_dbdgemm accumulates into a binned C; K split in two calls
+    equals one call (corroborate_rdgemm.c-style blocking)."""
+    import reproblas_amd as rb
+
+    M, N, K = 37, 29, 300
+    fold = 3
+    A = dg.wide(M * K, 95)
+    B = dg.wide(K * N, 96)
+    Cb_e = np.zeros(M * N * 2 * fold)
+    oracle().dbdgemm(fold, "C", "N", "N", M, N, K, 1.0, A, M, B, K, Cb_e, M)
+    Cb = np.zeros(M * N * 2 * fold)
+    k1 = 123
+    rb.dbdgemm(fold, "C", "N", "N", M, N, k1, 1.0, A, M, B, K, Cb, M)
+    rb.dbdgemm(fold, "C", "N", "N", M, N, K - k1, 1.0, A[M * k1:], M, B[k1:], K, Cb, M)
+    assert np.array_equal(Cb.view(np.uint64), Cb_e.view(np.uint64))
+
+
+@pytest.mark.slow
+def test_gemv_full_size(gpu):
+    """BASELINE scale 16384^2 column-major N and T, fold 3, beta in {0, 0.5}."""
+    import torch
+
+    import reproblas_amd as rb
+
+    n = 16384
+    A = dg.normal((n, n), 201)  # column-major n x n (transposed view of C order)
+    x = dg.normal(n, 202)
+    y0 = dg.normal(n, 203)
+    tA, tx = torch.from_numpy(A).to(gpu), torch.from_numpy(x).to(gpu)
+    O = oracle()
+    for trans in ("N", "T"):
+        for beta in (0.0, 0.5):
+            ty = torch.from_numpy(y0.copy()).to(gpu)
+            rb.rdgemv(trans, 1.0, tA, tx, beta, ty)
+            ye = y0.copy()
+            O.fn["rdgemv"](3, b"C", trans.encode(), n, n, 1.0, vp(A), n, vp(x), 1, beta, vp(ye), 1)
+            assert eq_arrays(ty.cpu().numpy(), ye), (trans, beta)
