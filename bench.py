@@ -1,0 +1,237 @@
+"""Benchmark of the binned FP64 path on one MI355X.
+
+Headline (the JSON line's `value`): reproducible (bit-exact) ddot, fold 3,
+N = 2^28 FP64 with wide dynamic range x = s*m*2^e (s = +-1, m ~ U[1,2),
+e ~ U{-40..40}), both operands resident in HBM; GB/s of algorithmic traffic
+(16 bytes per term). A "step" is one ddot over the whole vector pair.
+
+Also reported under "extra": dsum 2^20 (uniform), dsum / dnrm2 2^28, dgemv
+16384^2 (N and T), dgemm 4096^3 (GFLOP/s and fraction of the FP64 VALU issue
+rate), each as one timed configuration.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+# FP64 VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz (one op per lane per
+# clock; the 78.6 TFLOP/s figure counts an FMA as two)
+VALU_PEAK_OPS = 256 * 64 * 2.4e9
+METRIC = "bit-exact ddot GB/s"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def wide_gpu(n, seed, device):
+    """x = s*m*2^e on the GPU from a seeded generator."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    s = torch.randint(0, 2, (n,), generator=g, device=device, dtype=torch.int8).to(torch.float64).mul_(2).sub_(1)
+    m = torch.rand(n, generator=g, device=device, dtype=torch.float64).add_(1.0)
+    e = torch.randint(-40, 41, (n,), generator=g, device=device, dtype=torch.int32)
+    return torch.ldexp(m, e).mul_(s)
+
+
+def time_launches(fn, steps, warmup, stream):
+    """Average time per call of fn (which enqueues on `stream`), HIP events."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        fn()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / 1e3 / steps
+
+
+def bits(v):
+    return int(np.array([v]).view(np.uint64)[0])
+
+
+def cpu_baseline(x_sample, y_sample):
+    """The reference's own ddot (oracle/_ref, compiled from its sources) on the
+    host, single thread, on a bounded sample; falls back to the oracle port."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from cpu_libs import oracle, reference
+
+    lib = reference("avx")
+    kind = "reference"
+    if lib is None:
+        lib, kind = oracle(), "port"
+    n = len(x_sample)
+    reps, t_total, res = 0, 0.0, None
+    while t_total < 10.0 and reps < 5000:
+        t = time.perf_counter()
+        res = lib.rddot(3, x_sample, y_sample)
+        t_total += time.perf_counter() - t
+        reps += 1
+    gbs = 16.0 * n * reps / t_total / 1e9
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": kind,
+            "sample": f"rddot fold 3 on the first {n} terms of the same vectors, {reps} repetitions, "
+                      f"{t_total:.1f} s"}, res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--quick", action="store_true", help="headline only (no extra configurations)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(the multi-GPU reduction is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import reproblas_amd as rb
+
+    dev = torch.device("cuda:0")
+    L = rb.lib()
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    # ------------------------------------------------------------ headline --
+    n = 1 << 28
+    log(f"[bench] generating ddot operands N={n}")
+    x = wide_gpu(n, 1, dev)
+    y = wide_gpu(n, 2, dev)
+    out = torch.zeros(8, dtype=torch.float64, device=dev)
+
+    def ddot():
+        L.reproblas_hip_rddot_device(3, n, vp(x), 1, vp(y), 1, vp(out), sp)
+
+    log("[bench] timing ddot")
+    t = time_launches(ddot, args.steps, args.warmup, stream)
+    rb._check()
+    value = 16.0 * n / t / 1e9
+    log(f"[bench] ddot {t * 1e3:.3f} ms/call, {value:.1f} GB/s")
+
+    # bit-exactness spot check on a sample against the CPU reference
+    ns = 1 << 24
+    xs = x[:ns].cpu().numpy()
+    ys = y[:ns].cpu().numpy()
+    L.reproblas_hip_rddot_device(3, ns, vp(x), 1, vp(y), 1, vp(out[1:]), sp)
+    torch.cuda.synchronize()
+    gpu_sample = float(out[1].item())
+    cpu = None
+    ref_sample = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        cpu, ref_sample = cpu_baseline(xs, ys)
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic_blas1_ddot_2p28.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "reproBLAS_ddot fold=3 N=2^28 wide-range x=s*m*2^e, e in [-40,40], incX=incY=1",
+                   "fold": 3, "n": n, "bytes_per_step": 16 * n},
+        "roofline": {"bound": "hbm", "achieved": round(value, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(value / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "parity_sample": None if ref_sample is None else (bits(ref_sample) == bits(gpu_sample)),
+    }
+    del x, y
+    torch.cuda.empty_cache()
+
+    if not args.quick:
+        line["extra"] = extras(L, rb, dev, stream, sp, vp, args)
+    print(json.dumps(line), flush=True)
+
+
+def extras(L, rb, dev, stream, sp, vp, args):
+    import torch
+
+    ex = {}
+    steps = max(3, args.steps // 2)
+    out = torch.zeros(8, dtype=torch.float64, device=dev)
+
+    # dsum fold 3, N = 2^20 uniform(-1,1): launch/latency bound, report us
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x = torch.rand(1 << 20, generator=g, device=dev, dtype=torch.float64).mul_(2).sub_(1)
+    t = time_launches(lambda: L.reproblas_hip_rdsum_device(3, 1 << 20, vp(x), 1, vp(out), sp), 50, 5, stream)
+    ex["dsum_2p20"] = {"us_per_call": round(t * 1e6, 2), "GB/s": round(8 * (1 << 20) / t / 1e9, 1)}
+    log(f"[bench] dsum 2^20: {t * 1e6:.1f} us")
+
+    n = 1 << 28
+    x = wide_gpu(n, 3, dev)
+    for name, fn in (("dsum_2p28", lambda: L.reproblas_hip_rdsum_device(3, n, vp(x), 1, vp(out), sp)),
+                     ("dnrm2_2p28", lambda: L.reproblas_hip_rdnrm2_device(3, n, vp(x), 1, vp(out), sp))):
+        t = time_launches(fn, steps, 2, stream)
+        gbs = 8 * n / t / 1e9
+        ex[name] = {"ms": round(t * 1e3, 3), "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"[bench] {name}: {t * 1e3:.3f} ms, {gbs:.1f} GB/s")
+    del x
+    torch.cuda.empty_cache()
+
+    # dgemv 16384^2 column-major, fold 3, alpha 1, beta 0.5
+    m = 16384
+    A = torch.randn(m * m, generator=g, device=dev, dtype=torch.float64)
+    xv = torch.randn(m, generator=g, device=dev, dtype=torch.float64)
+    yv = torch.randn(m, generator=g, device=dev, dtype=torch.float64)
+    for tr in ("N", "T"):
+        fn = lambda: L.reproblas_hip_rdgemv_device(3, b"C", tr.encode(), m, m, 1.0, vp(A), m, vp(xv), 1, 0.5, vp(yv),
+                                                   1, sp)
+        t = time_launches(fn, steps, 2, stream)
+        byts = 8 * (m * m + m + 2 * m)
+        ex[f"dgemv_{tr}_16384"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
+                                   "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] dgemv {tr}: {t * 1e3:.3f} ms, {byts / t / 1e9:.1f} GB/s")
+    del A
+    torch.cuda.empty_cache()
+
+    # dgemm 4096^3 column-major NN, fold 3, alpha 1, beta 0
+    k = 4096
+    A = torch.randn(k * k, generator=g, device=dev, dtype=torch.float64)
+    B = torch.randn(k * k, generator=g, device=dev, dtype=torch.float64)
+    C = torch.zeros(k * k, device=dev, dtype=torch.float64)
+    fn = lambda: L.reproblas_hip_rdgemm_device(3, b"C", b"N", b"N", k, k, k, 1.0, vp(A), k, vp(B), k, 0.0, vp(C), k,
+                                               sp)
+    t = time_launches(fn, 2, 1, stream)
+    ops = 11.0 * k ** 3  # (4*fold-1) ops per product, the reference's model (bench_rdgemm.c:125-127)
+    ex["dgemm_NN_4096"] = {"ms": round(t * 1e3, 2), "GFLOP/s": round(ops / t / 1e9, 1),
+                           "valu_frac": round(ops / t / VALU_PEAK_OPS, 4),
+                           "products_per_s": round(k ** 3 / t, 1)}
+    log(f"[bench] dgemm: {t * 1e3:.2f} ms, {ops / t / 1e9:.1f} GFLOP/s (model ops), {ops / t / VALU_PEAK_OPS:.3f} of VALU")
+    return ex
+
+
+if __name__ == "__main__":
+    main()

@@ -200,34 +200,22 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
     }
     const double bound = (aalpha * ra) * cb;
     if (!(bound < lmin)) {
-      // exact max |fl(a*b)| of each output over the chunk, then raise
-      double mx[TM][TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) mx[i][j] = 0.0;
-      for (int k = 0; k < KC; ++k) {
-        double a[TM], b[TN];
-#pragma unroll
-        for (int i = 0; i < TM; ++i) a[i] = As[k][tr * TM + i];
-#pragma unroll
-        for (int j = 0; j < TN; ++j) b[j] = Bs[k][tc * TN + j];
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-          for (int j = 0; j < TN; ++j) mx[i][j] = fmax(mx[i][j], fabs(a[i] * b[j]));
-      }
+      // exact max |fl(a*b)| of each output over the chunk, then raise; one
+      // output at a time keeps this rare path's registers few
       lmin = __builtin_inf();
       anytop = false;
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
+          double mx = 0.0;
+#pragma unroll 1
+          for (int k = 0; k < KC; ++k) mx = fmax(mx, fabs(As[k][tr * TM + i] * Bs[k][tc * TN + j]));
           const int cur = dmindex(p[i][j][0]);
-          const int nI = dindex(mx[i][j]);
+          const int nI = dindex(mx);
           if (!nan_or_inf(p[i][j][0]) && nI < cur) raise_ic(fold, cur - nI, nI, p[i][j], c[i][j]);
-          const int now = dmindex(p[i][j][0]);
-          if (!nan_or_inf(p[i][j][0])) lmin = fmin(lmin, admit_limit(now));
+          const bool live = (i0 + tr * TM + i < g.M) && (j0 + tc * TN + j < g.N);
+          if (live && !nan_or_inf(p[i][j][0])) lmin = fmin(lmin, admit_limit(dmindex(p[i][j][0])));
           anytop |= dmindex0(p[i][j][0]);
         }
     }
@@ -241,7 +229,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
 
     // ---- deposits: 1 mul + (4F-2) add/or per product
     if (!anytop) {
-#pragma unroll 2
+#pragma unroll 1
       for (int k = 0; k < KC; ++k) {
         double a[TM], b[TN];
 #pragma unroll
@@ -254,6 +242,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
           for (int j = 0; j < TN; ++j) deposit(fold, false, a[i] * b[j], p[i][j]);
       }
     } else {
+#pragma unroll 1
       for (int k = 0; k < KC; ++k) {
         double a[TM], b[TN];
 #pragma unroll
@@ -331,15 +320,18 @@ bool launch_tm(bool ai, bool bk, const GemmArgs& g, hipStream_t s) {
   return RB_HIP(hipGetLastError());
 }
 
-int g_tiling = 0;  // 0: 4x4 outputs per lane, 1: 4x2, 2: 2x2
+int g_tiling = 1;  // 0: 4x4 outputs per lane, 1: 4x2, 2: 2x2
 
 template <int CAP, bool FIXED>
 bool launch_fold(bool ai, bool bk, const GemmArgs& g, hipStream_t s) {
-  const int t = FIXED ? g_tiling : 2;
-  switch (t) {
-    case 0: return launch_tm<CAP, FIXED, 4, 4>(ai, bk, g, s);
-    case 1: return launch_tm<CAP, FIXED, 4, 2>(ai, bk, g, s);
-    default: return launch_tm<CAP, FIXED, 2, 2>(ai, bk, g, s);
+  if constexpr (!FIXED) {
+    return launch_tm<CAP, FIXED, 2, 2>(ai, bk, g, s);  // folds 5..8: fewest registers
+  } else {
+    switch (g_tiling) {
+      case 0: return launch_tm<CAP, FIXED, 4, 4>(ai, bk, g, s);
+      case 1: return launch_tm<CAP, FIXED, 4, 2>(ai, bk, g, s);
+      default: return launch_tm<CAP, FIXED, 2, 2>(ai, bk, g, s);
+    }
   }
 }
 
@@ -369,7 +361,7 @@ bool gemm_common(int fold, char order, char transa, char transb, int64_t M, int6
   g.Cb = Cb;
   g.fold = fold;
   const int64_t chunks = (K + KC - 1) / KC;
-  double* mx = (double*)workspace(s, kSlotStageZ, sizeof(double) * (size_t)chunks * (M + N));
+  double* mx = (double*)workspace(s, kSlotGemmMax, sizeof(double) * (size_t)chunks * (M + N));
   if (!mx) return false;
   g.rowmax = mx;
   g.colmax = mx + chunks * M;

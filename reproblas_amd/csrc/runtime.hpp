@@ -63,6 +63,7 @@ enum WorkspaceSlot : int {
   kSlotStageZ = 4,
   kSlotBinnedOut = 5,
   kSlotBatch = 6,
+  kSlotGemmMax = 7,  // per-chunk row/column maxima of the gemm operands
   kSlotCount = 8,
 };
 

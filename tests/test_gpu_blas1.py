@@ -87,6 +87,8 @@ def test_distributions_device_operands(gpu, op, dist):
 def test_special_values(gpu, op, kind, n):
     import reproblas_amd as rb
 
+    if n < 2 and kind in ("both_inf", "inf_nan"):
+        pytest.skip("needs two special members")
     x = dg.special(kind, n, 5)
     y = dg.uniform(n, 6)
     r = gpu_call(rb, op, 3, x, y)
