@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configurations)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="time a plain device copy and a plain torch sum of 2^28 doubles (scales the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -109,6 +111,9 @@ def main():
     import reproblas_amd as rb
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        calibrate(dev, args)
+        return
     L = rb.lib()
     stream = torch.cuda.current_stream(dev)
     sp = ctypes.c_void_p(stream.cuda_stream)
@@ -173,6 +178,22 @@ def main():
     if not args.quick:
         line["extra"] = extras(L, rb, dev, stream, sp, vp, args)
     print(json.dumps(line), flush=True)
+
+
+def calibrate(dev, args):
+    """Reference kernels of known traffic: copy (8+8 bytes per element) and a
+    plain sum (8 bytes per element) of 2^28 doubles."""
+    import torch
+
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream(dev)
+    t_copy = time_launches(lambda: y.copy_(x), args.steps, args.warmup, stream)
+    t_sum = time_launches(lambda: x.sum(), args.steps, args.warmup, stream)
+    print(json.dumps({"copy_bytes_per_launch": 16 * n, "copy_ms": round(t_copy * 1e3, 4),
+                      "copy_GB/s": round(16 * n / t_copy / 1e9, 1), "sum_bytes_per_launch": 8 * n,
+                      "sum_ms": round(t_sum * 1e3, 4), "sum_GB/s": round(8 * n / t_sum / 1e9, 1)}), flush=True)
 
 
 def extras(L, rb, dev, stream, sp, vp, args):

@@ -77,6 +77,34 @@ int reproblas_hip_gemm_tiling_count(void);
 int reproblas_hip_gemm_set_tiling(int tiling);
 int reproblas_hip_gemm_get_tiling(void);
 
+/* Ragged batches (one persistent launch per group, optionally one hipGraph).
+ * A plan is built once on the host from host-side sizes and device operand
+ * pointers, then launched any number of times; every result is bit-identical
+ * to the corresponding single call (reproBLAS_rddot etc. / reproBLAS_rdgemm).
+ * Calls that allocate: create and add_* (plan tables); launches do not. */
+typedef struct reproblas_hip_batch reproblas_hip_batch;
+#define REPROBLAS_HIP_OP_SUM 0
+#define REPROBLAS_HIP_OP_ASUM 1
+#define REPROBLAS_HIP_OP_DOT 2
+#define REPROBLAS_HIP_OP_NRM2 3
+reproblas_hip_batch *reproblas_hip_batch_create(int fold);
+/* segment s of X (and Y for DOT) is [offsets[s], offsets[s+1]); offsets is a
+ * host array of nseg+1 entries; results (device, nseg doubles); chunk: terms
+ * per work item (0 for the default), a tuning knob only */
+int reproblas_hip_batch_add_segments(reproblas_hip_batch *b, int op, int64_t nseg, const int64_t *offsets,
+                                     const double *X, const double *Y, double *results, int64_t chunk);
+/* count gemms of one Order/TransA/TransB; per-gemm sizes, scalars and device
+ * pointers in host arrays; tiling as for reproblas_hip_gemm_set_tiling (-1:
+ * the current one) */
+int reproblas_hip_batch_add_gemms(reproblas_hip_batch *b, char Order, char TransA, char TransB, int count,
+                                  const int64_t *M, const int64_t *N, const int64_t *K, const double *alpha,
+                                  const double *const *A, const int64_t *lda, const double *const *B,
+                                  const int64_t *ldb, const double *beta, double *const *C, const int64_t *ldc,
+                                  int tiling);
+int reproblas_hip_batch_launch(reproblas_hip_batch *b, void *stream);
+int reproblas_hip_batch_graph_launch(reproblas_hip_batch *b, void *stream);
+void reproblas_hip_batch_destroy(reproblas_hip_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

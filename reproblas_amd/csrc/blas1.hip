@@ -48,8 +48,6 @@ struct Blas1Args {
   int64_t incx;
   const double* y;
   int64_t incy;
-  int64_t tiles;       // wave tiles covering n
-  int64_t full_tiles;  // tiles with no out-of-range element
   double* partials;    // gridDim.x records of (2*fold+1) doubles
   unsigned* ticket;
   const double* state_in;
@@ -87,27 +85,54 @@ __device__ __forceinline__ double plain_term(double x, double y) {
   return fabs(x);
 }
 
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// A strided vector range walked in wave tiles of 64*U terms.
+struct Span {
+  const double* x;
+  int64_t incx;
+  const double* y;  // second operand (dot) or null
+  int64_t incy;
+  int64_t n;
+  int64_t tiles;       // tiles covering n
+  int64_t full_tiles;  // tiles with no out-of-range element
+};
+
+template <int U>
+__device__ __forceinline__ Span make_span(const double* x, int64_t incx, const double* y, int64_t incy, int64_t n) {
+  Span sp;
+  sp.x = x;
+  sp.incx = incx;
+  sp.y = y;
+  sp.incy = incy;
+  sp.n = n;
+  sp.tiles = (n + 64 * U - 1) / (64 * U);
+  sp.full_tiles = n / (64 * U);
+  return sp;
+}
+
 // Loads the U raw terms of tile t for this lane. MODE 0: unit stride with
 // 16-byte vector loads (operands 16-byte aligned); MODE 1: unit stride, 8-byte
 // loads; MODE 2: arbitrary stride. Out-of-range terms read as 0, which every
 // deposit leaves unchanged.
 template <int T, int U, int MODE>
-__device__ __forceinline__ void load_tile(const Blas1Args& a, int64_t t, int lane, bool full, double (&vx)[U],
-                                          double (&vy)[U]) {
+__device__ __forceinline__ void load_tile(const Span& a, int64_t t, int lane, double (&vx)[U], double (&vy)[U]) {
   const int64_t base = t * (64 * U);
+  const bool full = t < a.full_tiles;
   if (MODE == 0 && full) {
-    const double2* xv = reinterpret_cast<const double2*>(a.x + base);
+    // streaming 16-byte loads (non-temporal: every byte is read once)
+    const d2* xv = reinterpret_cast<const d2*>(a.x + base);
 #pragma unroll
     for (int k = 0; k < U / 2; ++k) {
-      const double2 d = xv[k * 64 + lane];
+      const d2 d = __builtin_nontemporal_load(xv + k * 64 + lane);
       vx[2 * k] = d.x;
       vx[2 * k + 1] = d.y;
     }
     if (T == kDot) {
-      const double2* yv = reinterpret_cast<const double2*>(a.y + base);
+      const d2* yv = reinterpret_cast<const d2*>(a.y + base);
 #pragma unroll
       for (int k = 0; k < U / 2; ++k) {
-        const double2 d = yv[k * 64 + lane];
+        const d2 d = __builtin_nontemporal_load(yv + k * 64 + lane);
         vy[2 * k] = d.x;
         vy[2 * k + 1] = d.y;
       }
@@ -198,11 +223,11 @@ __device__ __forceinline__ void wave_merge_general(int fold, State<CAP>& s) {
 // both infinities occur, else the infinity that occurs. Terms as the
 // reference's infinity path forms them.
 template <int T, int U, int MODE>
-__device__ double wave_term_class(const Blas1Args& a, int64_t first, int64_t step, int lane) {
+__device__ double wave_term_class(const Span& a, int64_t first, int64_t step, int lane) {
   bool nan = false, pinf = false, ninf = false;
   for (int64_t t = first; t < a.tiles; t += step) {
     double vx[U], vy[U];
-    load_tile<T, U, MODE>(a, t, lane, t < a.full_tiles, vx, vy);
+    load_tile<T, U, MODE>(a, t, lane, vx, vy);
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const double v = plain_term<T>(vx[k], T == kDot ? vy[k] : 0.0);
@@ -216,34 +241,35 @@ __device__ double wave_term_class(const Blas1Args& a, int64_t first, int64_t ste
   return any_p ? __builtin_inf() : -__builtin_inf();
 }
 
-template <int CAP, bool FIXED, int T, int U, int MODE>
-__global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
-  const int fold = FIXED ? CAP : a.fold;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + wave;
-  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
-
+// One wave reduces the tiles t0, t0 + tstep, ... of a span into `s` (every
+// lane ends with the wave's value; a poisoned value is replaced by the class
+// of the terms).
+template <int CAP, int T, int U, int MODE>
+__device__ __forceinline__ void wave_accumulate(const Span& a, int fold, int64_t t0, int64_t tstep, int lane,
+                                                State<CAP>& s) {
   // Every wave starts at index 51, the reference's representation of an
   // empty sum (dmdupdate of 0.0 on a zero value), so every tile deposits.
-  State<CAP> s;
   int idx = kMaxIndex;
   set_bins(fold, idx, s.p, s.c);
   double lim = admit_limit(idx);
   double scl = 0.0, inv = 0.0;  // nrm2: current scale of the wave and 1/scale
   int dep = 0;
 
-  for (int64_t t = wave_gid; t < a.tiles; t += nwaves) {
-    double vx[U], vy[U];
-    load_tile<T, U, MODE>(a, t, lane, t < a.full_tiles, vx, vy);
+  // the next tile's loads are issued before the current tile is deposited
+  double cx[U], cy[U];
+  if (t0 < a.tiles) load_tile<T, U, MODE>(a, t0, lane, cx, cy);
+  for (int64_t t = t0; t < a.tiles; t += tstep) {
+    double nx[U], ny[U];
+    const int64_t tn = t + tstep;
+    if (tn < a.tiles) load_tile<T, U, MODE>(a, tn, lane, nx, ny);
     double v[U];
     double m = 0.0;
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      if (T == kSum) v[k] = vx[k];
-      if (T == kASum) v[k] = fabs(vx[k]);
-      if (T == kDot) v[k] = vx[k] * vy[k];
-      if (T == kSSq) v[k] = vx[k];
+      if (T == kSum) v[k] = cx[k];
+      if (T == kASum) v[k] = fabs(cx[k]);
+      if (T == kDot) v[k] = cx[k] * cy[k];
+      if (T == kSSq) v[k] = cx[k];
       m = fmax(m, fabs(v[k]));
     }
     bool need;
@@ -296,6 +322,11 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
       }
     }
     dep += U;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      cx[k] = nx[k];
+      if (T == kDot) cy[k] = ny[k];
+    }
   }
   renorm(fold, s.p, s.c);
   s.scl = scl;
@@ -316,8 +347,20 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
     }
   }
   if (nan_or_inf(uniform_double(s.p[0]))) {
-    s.p[0] = wave_term_class<T, U, MODE>(a, wave_gid, nwaves, lane);
+    s.p[0] = wave_term_class<T, U, MODE>(a, t0, tstep, lane);
   }
+}
+
+template <int CAP, bool FIXED, int T, int U, int MODE>
+__global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + wave;
+  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+  const Span sp = make_span<U>(a.x, a.incx, a.y, a.incy, a.n);
+  State<CAP> s;
+  wave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s);
 
   // Workgroup merge through LDS, then one partial per workgroup.
   __shared__ double lds[kWaves][2 * CAP + 1];
@@ -426,17 +469,103 @@ bool launch_mode(int mode, const Blas1Args& a, int blocks, hipStream_t s) {
   return RB_HIP(hipGetLastError());
 }
 
+// Terms per lane per tile: 16 for the one-operand reductions, 8 for dot, so
+// every tile moves 8 KiB per wave.
+constexpr int kUOne = 16;
+constexpr int kUDot = 8;
+
 template <int CAP, bool FIXED>
 bool launch_term(Term term, int mode, const Blas1Args& a, int blocks, hipStream_t s) {
   switch (term) {
-    case kSum: return launch_mode<CAP, FIXED, kSum, 8>(mode, a, blocks, s);
-    case kASum: return launch_mode<CAP, FIXED, kASum, 8>(mode, a, blocks, s);
-    case kDot: return launch_mode<CAP, FIXED, kDot, 8>(mode, a, blocks, s);
-    default: return launch_mode<CAP, FIXED, kSSq, 8>(mode, a, blocks, s);
+    case kSum: return launch_mode<CAP, FIXED, kSum, kUOne>(mode, a, blocks, s);
+    case kASum: return launch_mode<CAP, FIXED, kASum, kUOne>(mode, a, blocks, s);
+    case kDot: return launch_mode<CAP, FIXED, kDot, kUDot>(mode, a, blocks, s);
+    default: return launch_mode<CAP, FIXED, kSSq, kUOne>(mode, a, blocks, s);
   }
 }
 
-constexpr int kTileU = 8;
+// ------------------------------------------------------- segment batches --
+// A ragged batch of sums/dots over CSR segments of one vector pair. The host
+// cuts every segment into chunks of at most `chunk` terms; each wave reduces
+// whole chunks (grid-stride over the chunk table) into one binned partial per
+// chunk; a second kernel merges each segment's chunk partials in order and
+// converts. Chunk boundaries do not change the bits.
+struct SegArgs {
+  const double* x;
+  const double* y;
+  const SegChunk* chunks;
+  int64_t nchunks;
+  double* partials;  // nchunks records of (2*fold+1) doubles
+  const int64_t* first_chunk;  // segment s owns chunks [first_chunk[s], first_chunk[s+1])
+  int64_t nseg;
+  double* results;
+  int fold;
+};
+
+template <int CAP, bool FIXED, int T, int U>
+__global__ __launch_bounds__(kThreads) void seg_chunk_kernel(SegArgs a) {
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+  for (int64_t c = wave_gid; c < a.nchunks; c += nwaves) {
+    const SegChunk ch = a.chunks[c];
+    const double* xb = a.x + ch.begin;
+    const double* yb = (T == kDot) ? a.y + ch.begin : nullptr;
+    const Span sp = make_span<U>(xb, 1, yb, 1, ch.end - ch.begin);
+    const bool aligned = ((uintptr_t)xb % 16 == 0) && (T != kDot || (uintptr_t)yb % 16 == 0);
+    State<CAP> st;
+    if (aligned)
+      wave_accumulate<CAP, T, U, 0>(sp, fold, 0, 1, lane, st);
+    else
+      wave_accumulate<CAP, T, U, 1>(sp, fold, 0, 1, lane, st);
+    if (lane == 0) store_state(a.partials + c * (2 * fold + 1), fold, st);
+  }
+}
+
+template <int CAP, bool FIXED, int T>
+__global__ __launch_bounds__(kThreads) void seg_finish_kernel(SegArgs a) {
+  const int fold = FIXED ? CAP : a.fold;
+  const int64_t sg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sg >= a.nseg) return;
+  const int64_t c0 = a.first_chunk[sg], c1 = a.first_chunk[sg + 1];
+  if (c0 == c1) {
+    a.results[sg] = 0.0;  // empty segment: the reference's N = 0 result
+    return;
+  }
+  State<CAP> acc;
+  load_state(a.partials + c0 * (2 * fold + 1), fold, acc);
+  for (int64_t c = c0 + 1; c < c1; ++c) {
+    State<CAP> o;
+    load_state(a.partials + c * (2 * fold + 1), fold, o);
+    merge<CAP, T>(fold, acc, o);
+  }
+  renorm(fold, acc.p, acc.c);
+  const double v = to_double(fold, acc.p, acc.c);
+  if (T == kSSq) {
+    const double scale = nan_or_inf(acc.p[0]) ? 1.0 : acc.scl;
+    a.results[sg] = scale * sqrt(v);
+  } else {
+    a.results[sg] = v;
+  }
+}
+
+template <int CAP, bool FIXED, int T, int U>
+bool launch_seg(const SegArgs& a, int grid, hipStream_t s) {
+  if (a.nchunks > 0) seg_chunk_kernel<CAP, FIXED, T, U><<<grid, kThreads, 0, s>>>(a);
+  if (a.nseg > 0) seg_finish_kernel<CAP, FIXED, T><<<(unsigned)((a.nseg + 255) / 256), 256, 0, s>>>(a);
+  return RB_HIP(hipGetLastError());
+}
+
+template <int CAP, bool FIXED>
+bool launch_seg_term(Term term, const SegArgs& a, int grid, hipStream_t s) {
+  switch (term) {
+    case kSum: return launch_seg<CAP, FIXED, kSum, kUOne>(a, grid, s);
+    case kASum: return launch_seg<CAP, FIXED, kASum, kUOne>(a, grid, s);
+    case kDot: return launch_seg<CAP, FIXED, kDot, kUDot>(a, grid, s);
+    default: return launch_seg<CAP, FIXED, kSSq, kUOne>(a, grid, s);
+  }
+}
 
 }  // namespace
 
@@ -457,9 +586,8 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
   a.incx = incx;
   a.y = y;
   a.incy = incy;
-  const int64_t per_tile = 64 * kTileU;
-  a.tiles = (a.n + per_tile - 1) / per_tile;
-  a.full_tiles = a.n / per_tile;
+  const int64_t per_tile = 64 * (term == kDot ? kUDot : kUOne);
+  const int64_t tiles = (a.n + per_tile - 1) / per_tile;
   a.state_in = state_in;
   a.scale_in = scale_in;
   a.scale_in_dev = scale_in_dev;
@@ -469,7 +597,7 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
   a.fold = fold;
 
   // grid: enough waves to keep every CU streaming, at most one wave per tile
-  int64_t blocks = (a.tiles + kWaves - 1) / kWaves;
+  int64_t blocks = (tiles + kWaves - 1) / kWaves;
   const int64_t cap = (int64_t)dev->cus * 8;
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
@@ -490,6 +618,31 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
     case 3: return launch_term<3, true>(term, mode, a, (int)blocks, s);
     case 4: return launch_term<4, true>(term, mode, a, (int)blocks, s);
     default: return launch_term<8, false>(term, mode, a, (int)blocks, s);
+  }
+}
+
+bool segments(int fold, Term term, const double* x, const double* y, const SegChunk* chunks, int64_t nchunks,
+              double* partials, const int64_t* first_chunk, int64_t nseg, double* results, int grid, hipStream_t s) {
+  if (!fold_supported(fold)) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    return false;
+  }
+  SegArgs a;
+  a.x = x;
+  a.y = y;
+  a.chunks = chunks;
+  a.nchunks = nchunks;
+  a.partials = partials;
+  a.first_chunk = first_chunk;
+  a.nseg = nseg;
+  a.results = results;
+  a.fold = fold;
+  if (grid < 1) grid = 1;
+  switch (fold) {
+    case 2: return launch_seg_term<2, true>(term, a, grid, s);
+    case 3: return launch_seg_term<3, true>(term, a, grid, s);
+    case 4: return launch_seg_term<4, true>(term, a, grid, s);
+    default: return launch_seg_term<8, false>(term, a, grid, s);
   }
 }
 

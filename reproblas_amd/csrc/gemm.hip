@@ -35,22 +35,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int KC = 32;
 
-struct GemmArgs {
-  int64_t M, N, K;
-  const double* A;
-  int64_t sai, sak;  // A(i,k) = A[i*sai + k*sak]
-  const double* B;
-  int64_t sbk, sbj;  // B(k,j) = B[k*sbk + j*sbj]
-  double alpha;
-  double* C;  // double output (when Cb is null)
-  int64_t sci, scj;
-  double beta;
-  double* Cb;  // binned output accumulated into, element (i,j) at (i*sbi + j*sbj2)*2F
-  int64_t cbi, cbj;
-  const double* rowmax;  // [chunks][M] max |A(i,k)| over the chunk
-  const double* colmax;  // [chunks][N] max |B(k,j)| over the chunk
-  int fold;
-};
+
 
 // ---- int-carry variants of renorm / raise (carries are small integers) ----
 template <int CAP>
@@ -125,16 +110,16 @@ __device__ __forceinline__ void beta_init(int fold, double beta, double yv, doub
 }
 
 // ------------------------------------------------------------------ kernel --
+constexpr int kPad = 2;  // keeps LDS rows 16-byte aligned, spreads staging writes over banks
+
+// One (16*TM) x (16*TN) tile of C at (i0, j0), all of K.
 template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK>
-__global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
+__device__ __forceinline__ void gemm_tile(const GemmDesc& g, int64_t i0, int64_t j0,
+                                          double (&As)[KC][16 * TM + kPad], double (&Bs)[KC][16 * TN + kPad]) {
   constexpr int BM = 16 * TM, BN = 16 * TN;
-  constexpr int PAD = 2;  // keeps rows 16-byte aligned, spreads staging writes over banks
-  __shared__ __attribute__((aligned(16))) double As[KC][BM + PAD];
-  __shared__ __attribute__((aligned(16))) double Bs[KC][BN + PAD];
   const int fold = FIXED ? CAP : g.fold;
   const int tid = threadIdx.x;
   const int tr = tid / 16, tc = tid % 16;
-  const int64_t i0 = (int64_t)blockIdx.y * BM, j0 = (int64_t)blockIdx.x * BN;
 
   double p[TM][TN][CAP];
   int c[TM][TN][CAP];
@@ -310,8 +295,49 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
   }
 }
 
+template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK>
+__global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc g) {
+  __shared__ __attribute__((aligned(16))) double As[KC][16 * TM + kPad];
+  __shared__ __attribute__((aligned(16))) double Bs[KC][16 * TN + kPad];
+  gemm_tile<CAP, FIXED, TM, TN, AI, BK>(g, (int64_t)blockIdx.y * 16 * TM, (int64_t)blockIdx.x * 16 * TN, As, Bs);
+}
+
+// Ragged gemm batch: a persistent grid walks a tile list (gemm, tile row,
+// tile column), heaviest K first, each workgroup taking every gridDim.x-th.
+template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK>
+__global__ __launch_bounds__(kThreads) void gemm_batch_kernel(const GemmDesc* descs, const GemmTile* tiles,
+                                                              int64_t ntiles) {
+  __shared__ __attribute__((aligned(16))) double As[KC][16 * TM + kPad];
+  __shared__ __attribute__((aligned(16))) double Bs[KC][16 * TN + kPad];
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const GemmTile tl = tiles[t];
+    const GemmDesc g = descs[tl.gemm];
+    gemm_tile<CAP, FIXED, TM, TN, AI, BK>(g, (int64_t)tl.ti * 16 * TM, (int64_t)tl.tj * 16 * TN, As, Bs);
+    __syncthreads();  // the next tile reuses the LDS tiles
+  }
+}
+
+// Chunk maxima for every gemm of a batch (blockIdx.y = gemm).
+__global__ __launch_bounds__(256) void chunk_max_batch_kernel(const GemmDesc* descs) {
+  const GemmDesc g = descs[blockIdx.y];
+  const int64_t chunks = (g.K + KC - 1) / KC;
+  const int64_t total = (g.M + g.N) * chunks;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const bool isA = t < g.M * chunks;
+    const int64_t u = isA ? t : t - g.M * chunks;
+    const int64_t rows = isA ? g.M : g.N;
+    const int64_t r = u % rows, ch = u / rows;
+    const int64_t k0 = ch * KC, k1 = (k0 + KC < g.K) ? k0 + KC : g.K;
+    const double* X = isA ? g.A : g.B;
+    const int64_t s_row = isA ? g.sai : g.sbj, s_k = isA ? g.sak : g.sbk;
+    double m = 0.0;
+    for (int64_t k = k0; k < k1; ++k) m = fmax(m, fabs(X[r * s_row + k * s_k]));
+    (isA ? const_cast<double*>(g.rowmax) : const_cast<double*>(g.colmax))[ch * rows + r] = m;
+  }
+}
+
 template <int CAP, bool FIXED, int TM, int TN>
-bool launch_tm(bool ai, bool bk, const GemmArgs& g, hipStream_t s) {
+bool launch_tm(bool ai, bool bk, const GemmDesc& g, hipStream_t s) {
   dim3 grid((unsigned)((g.N + 16 * TN - 1) / (16 * TN)), (unsigned)((g.M + 16 * TM - 1) / (16 * TM)));
   if (ai && bk) gemm_kernel<CAP, FIXED, TM, TN, true, true><<<grid, kThreads, 0, s>>>(g);
   else if (ai) gemm_kernel<CAP, FIXED, TM, TN, true, false><<<grid, kThreads, 0, s>>>(g);
@@ -320,14 +346,26 @@ bool launch_tm(bool ai, bool bk, const GemmArgs& g, hipStream_t s) {
   return RB_HIP(hipGetLastError());
 }
 
+template <int CAP, bool FIXED, int TM, int TN>
+bool launch_batch_tm(bool ai, bool bk, const GemmDesc* d, const GemmTile* t, int64_t n, int grid, hipStream_t s) {
+  if (ai && bk) gemm_batch_kernel<CAP, FIXED, TM, TN, true, true><<<grid, kThreads, 0, s>>>(d, t, n);
+  else if (ai) gemm_batch_kernel<CAP, FIXED, TM, TN, true, false><<<grid, kThreads, 0, s>>>(d, t, n);
+  else if (bk) gemm_batch_kernel<CAP, FIXED, TM, TN, false, true><<<grid, kThreads, 0, s>>>(d, t, n);
+  else gemm_batch_kernel<CAP, FIXED, TM, TN, false, false><<<grid, kThreads, 0, s>>>(d, t, n);
+  return RB_HIP(hipGetLastError());
+}
+
 int g_tiling = 1;  // 0: 4x4 outputs per lane, 1: 4x2, 2: 2x2
 
+// Tiling actually used for a fold (folds without a dedicated instance use 2x2).
+int effective_tiling(int tiling, int fold) { return (fold >= 2 && fold <= 4) ? tiling : 2; }
+
 template <int CAP, bool FIXED>
-bool launch_fold(bool ai, bool bk, const GemmArgs& g, hipStream_t s) {
+bool launch_fold(int tiling, bool ai, bool bk, const GemmDesc& g, hipStream_t s) {
   if constexpr (!FIXED) {
     return launch_tm<CAP, FIXED, 2, 2>(ai, bk, g, s);  // folds 5..8: fewest registers
   } else {
-    switch (g_tiling) {
+    switch (tiling) {
       case 0: return launch_tm<CAP, FIXED, 4, 4>(ai, bk, g, s);
       case 1: return launch_tm<CAP, FIXED, 4, 2>(ai, bk, g, s);
       default: return launch_tm<CAP, FIXED, 2, 2>(ai, bk, g, s);
@@ -335,17 +373,54 @@ bool launch_fold(bool ai, bool bk, const GemmArgs& g, hipStream_t s) {
   }
 }
 
-bool gemm_common(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
-                 const double* A, int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc, double beta,
-                 double* Cb, int64_t ldcb, hipStream_t s) {
+template <int CAP, bool FIXED>
+bool launch_batch_fold(int tiling, bool ai, bool bk, const GemmDesc* d, const GemmTile* t, int64_t n, int grid,
+                       hipStream_t s) {
+  if constexpr (!FIXED) {
+    return launch_batch_tm<CAP, FIXED, 2, 2>(ai, bk, d, t, n, grid, s);
+  } else {
+    switch (tiling) {
+      case 0: return launch_batch_tm<CAP, FIXED, 4, 4>(ai, bk, d, t, n, grid, s);
+      case 1: return launch_batch_tm<CAP, FIXED, 4, 2>(ai, bk, d, t, n, grid, s);
+      default: return launch_batch_tm<CAP, FIXED, 2, 2>(ai, bk, d, t, n, grid, s);
+    }
+  }
+}
+
+bool gemm_common(const GemmDesc& g0, hipStream_t s) {
+  const int fold = g0.fold;
   if (!fold_supported(fold)) {
     set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
     return false;
   }
+  GemmDesc g = g0;
+  const int64_t chunks = (g.K + KC - 1) / KC;
+  double* mx = (double*)workspace(s, kSlotGemmMax, sizeof(double) * (size_t)chunks * (g.M + g.N));
+  if (!mx) return false;
+  g.rowmax = mx;
+  g.colmax = mx + chunks * g.M;
+  chunk_max_kernel<<<(unsigned)((g.M * chunks + 255) / 256), 256, 0, s>>>(g.M, g.K, g.A, g.sai, g.sak, mx);
+  chunk_max_kernel<<<(unsigned)((g.N * chunks + 255) / 256), 256, 0, s>>>(g.N, g.K, g.B, g.sbj, g.sbk,
+                                                                          mx + chunks * g.M);
+  if (!RB_HIP(hipGetLastError())) return false;
+  const bool ai = (g.sai == 1), bk = (g.sbk == 1);
+  switch (fold) {
+    case 2: return launch_fold<2, true>(g_tiling, ai, bk, g, s);
+    case 3: return launch_fold<3, true>(g_tiling, ai, bk, g, s);
+    case 4: return launch_fold<4, true>(g_tiling, ai, bk, g, s);
+    default: return launch_fold<8, false>(g_tiling, ai, bk, g, s);
+  }
+}
+
+}  // namespace
+
+GemmDesc make_gemm_desc(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                        double* Cb, int64_t ldcb) {
   const bool rm = (order == 'r' || order == 'R');
   const bool na = (transa == 'n' || transa == 'N');
   const bool nb = (transb == 'n' || transb == 'N');
-  GemmArgs g;
+  GemmDesc g;
   g.M = M;
   g.N = N;
   g.K = K;
@@ -359,25 +434,37 @@ bool gemm_common(int fold, char order, char transa, char transb, int64_t M, int6
   g.C = C;
   g.beta = beta;
   g.Cb = Cb;
+  g.rowmax = g.colmax = nullptr;
   g.fold = fold;
-  const int64_t chunks = (K + KC - 1) / KC;
-  double* mx = (double*)workspace(s, kSlotGemmMax, sizeof(double) * (size_t)chunks * (M + N));
-  if (!mx) return false;
-  g.rowmax = mx;
-  g.colmax = mx + chunks * M;
-  chunk_max_kernel<<<(unsigned)((M * chunks + 255) / 256), 256, 0, s>>>(M, K, A, g.sai, g.sak, mx);
-  chunk_max_kernel<<<(unsigned)((N * chunks + 255) / 256), 256, 0, s>>>(N, K, B, g.sbj, g.sbk, mx + chunks * M);
-  if (!RB_HIP(hipGetLastError())) return false;
-  const bool ai = (g.sai == 1), bk = (g.sbk == 1);
-  switch (fold) {
-    case 2: return launch_fold<2, true>(ai, bk, g, s);
-    case 3: return launch_fold<3, true>(ai, bk, g, s);
-    case 4: return launch_fold<4, true>(ai, bk, g, s);
-    default: return launch_fold<8, false>(ai, bk, g, s);
-  }
+  return g;
 }
 
-}  // namespace
+void gemm_tile_shape(int tiling, int fold, int* bm, int* bn) {
+  const int t = effective_tiling(tiling, fold);
+  *bm = 16 * (t == 2 ? 2 : 4);
+  *bn = 16 * (t == 0 ? 4 : 2);
+}
+
+bool gemm_batch(int fold, int tiling, const GemmDesc* d_descs, const GemmDesc& desc0, int count, int64_t max_work,
+                const GemmTile* d_tiles, int64_t ntiles, int grid, hipStream_t s) {
+  if (!fold_supported(fold)) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    return false;
+  }
+  if (count <= 0 || ntiles <= 0) return true;
+  int64_t mgrid = (max_work + 255) / 256;
+  if (mgrid > 1024) mgrid = 1024;
+  if (mgrid < 1) mgrid = 1;
+  chunk_max_batch_kernel<<<dim3((unsigned)mgrid, (unsigned)count), 256, 0, s>>>(d_descs);
+  if (!RB_HIP(hipGetLastError())) return false;
+  const bool ai = (desc0.sai == 1), bk = (desc0.sbk == 1);
+  switch (fold) {
+    case 2: return launch_batch_fold<2, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
+    case 3: return launch_batch_fold<3, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
+    case 4: return launch_batch_fold<4, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
+    default: return launch_batch_fold<8, false>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
+  }
+}
 
 int gemm_tiling_count() { return 3; }
 bool gemm_set_tiling(int t) {
@@ -393,7 +480,8 @@ int gemm_get_tiling() { return g_tiling; }
 bool dbdgemm(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
              const double* A, int64_t lda, const double* B, int64_t ldb, double* Cb, int64_t ldc, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0 || alpha == 0.0) return true;  // dbdgemm.ccog:77-80
-  return gemm_common(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, nullptr, 0, 0.0, Cb, ldc, s);
+  return gemm_common(
+      make_gemm_desc(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, 0.0, nullptr, 0, Cb, ldc), s);
 }
 
 bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
@@ -412,7 +500,8 @@ bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, in
     }
     return true;
   }
-  return gemm_common(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, C, ldc, beta, nullptr, 0, s);
+  return gemm_common(
+      make_gemm_desc(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0), s);
 }
 
 }  // namespace rb

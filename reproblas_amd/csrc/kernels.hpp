@@ -25,6 +25,17 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
            const double* state_in, double scale_in, const double* scale_in_dev, double* state_out,
            double* scale_out, double* result, hipStream_t s);
 
+// Ragged batch of BLAS1 reductions over CSR segments of x (and y for dot),
+// unit stride. chunks[c] = [begin, end) element range (at most a few
+// thousand terms); segment s owns chunks [first_chunk[s], first_chunk[s+1]).
+// partials: nchunks*(2*fold+1) doubles of scratch; results: nseg doubles.
+struct SegChunk {
+  int64_t begin;
+  int64_t end;
+};
+bool segments(int fold, Term term, const double* x, const double* y, const SegChunk* chunks, int64_t nchunks,
+              double* partials, const int64_t* first_chunk, int64_t nseg, double* results, int grid, hipStream_t s);
+
 // Elementwise binned helpers on device arrays of binned values.
 // dbdbaddv: Y[i] += X[i] for i < n (stride incX/incY in binned elements).
 bool dbdbaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int64_t incY, hipStream_t s);
@@ -54,6 +65,41 @@ bool rdgemv_device(int fold, char order, char trans, int64_t M, int64_t N, doubl
 bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
                    hipStream_t s);
+
+// One gemm on device operands in the kernels' terms:
+//   op(A)(i,k) = A[i*sai + k*sak], op(B)(k,j) = B[k*sbk + j*sbj],
+//   C(i,j) = C[i*sci + j*scj] (double output, beta applied) or, when Cb is
+//   set, the binned output Cb[(i*cbi + j*cbj)*2*fold..] accumulated into.
+// rowmax/colmax: scratch of ceil(K/32)*M and ceil(K/32)*N doubles.
+struct GemmDesc {
+  int64_t M, N, K;
+  const double* A;
+  int64_t sai, sak;
+  const double* B;
+  int64_t sbk, sbj;
+  double alpha;
+  double* C;
+  int64_t sci, scj;
+  double beta;
+  double* Cb;
+  int64_t cbi, cbj;
+  const double* rowmax;
+  const double* colmax;
+  int fold;
+};
+struct GemmTile {
+  int32_t gemm, ti, tj, pad;
+};
+// Fills a GemmDesc from BLAS arguments (Order/Trans characters).
+GemmDesc make_gemm_desc(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                        double* Cb, int64_t ldcb);
+// Tile edge of the batch kernel for a tiling: rows 16*TM, columns 16*TN.
+void gemm_tile_shape(int tiling, int fold, int* bm, int* bn);
+// Ragged gemm batch: all descs share one fold and one layout (A/B contiguity
+// flags taken from descs[0]); tiles list (gemm, tile row, tile column).
+bool gemm_batch(int fold, int tiling, const GemmDesc* d_descs, const GemmDesc& desc0, int count, int64_t max_work,
+                const GemmTile* d_tiles, int64_t ntiles, int grid, hipStream_t s);
 
 // gemm tilings (outputs per lane 4x4, 4x2, 2x2); all give identical bits
 int gemm_tiling_count();
