@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configurations)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--only", choices=["dsum_2p20", "dsum_2p28", "dnrm2_2p28", "dgemv", "dgemm", "ragged_batch", "mlp"],
+                    help="run one extra configuration alone and print its record (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy and a plain torch sum of 2^28 doubles (scales the HBM counters)")
     args = ap.parse_args()
@@ -118,6 +120,9 @@ def main():
     stream = torch.cuda.current_stream(dev)
     sp = ctypes.c_void_p(stream.cuda_stream)
     vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    if args.only:
+        print(json.dumps({args.only: extras(L, rb, dev, stream, sp, vp, args, only=args.only)[args.only]}), flush=True)
+        return
 
     # ------------------------------------------------------------ headline --
     n = 1 << 28
@@ -196,62 +201,154 @@ def calibrate(dev, args):
                       "sum_ms": round(t_sum * 1e3, 4), "sum_GB/s": round(8 * n / t_sum / 1e9, 1)}), flush=True)
 
 
-def extras(L, rb, dev, stream, sp, vp, args):
+def extras(L, rb, dev, stream, sp, vp, args, only=None):
     import torch
 
     ex = {}
+    want = lambda name: only is None or only == name
     steps = max(3, args.steps // 2)
     out = torch.zeros(8, dtype=torch.float64, device=dev)
 
-    # dsum fold 3, N = 2^20 uniform(-1,1): launch/latency bound, report us
     g = torch.Generator(device=dev)
     g.manual_seed(0)
-    x = torch.rand(1 << 20, generator=g, device=dev, dtype=torch.float64).mul_(2).sub_(1)
-    t = time_launches(lambda: L.reproblas_hip_rdsum_device(3, 1 << 20, vp(x), 1, vp(out), sp), 50, 5, stream)
-    ex["dsum_2p20"] = {"us_per_call": round(t * 1e6, 2), "GB/s": round(8 * (1 << 20) / t / 1e9, 1)}
-    log(f"[bench] dsum 2^20: {t * 1e6:.1f} us")
+    if want("dsum_2p20"):
+        # dsum fold 3, N = 2^20 uniform(-1,1): launch/latency bound, report us
+        x = torch.rand(1 << 20, generator=g, device=dev, dtype=torch.float64).mul_(2).sub_(1)
+        t = time_launches(lambda: L.reproblas_hip_rdsum_device(3, 1 << 20, vp(x), 1, vp(out), sp), 50, 5, stream)
+        ex["dsum_2p20"] = {"us_per_call": round(t * 1e6, 2), "GB/s": round(8 * (1 << 20) / t / 1e9, 1)}
+        log(f"[bench] dsum 2^20: {t * 1e6:.1f} us")
 
-    n = 1 << 28
-    x = wide_gpu(n, 3, dev)
-    for name, fn in (("dsum_2p28", lambda: L.reproblas_hip_rdsum_device(3, n, vp(x), 1, vp(out), sp)),
-                     ("dnrm2_2p28", lambda: L.reproblas_hip_rdnrm2_device(3, n, vp(x), 1, vp(out), sp))):
-        t = time_launches(fn, steps, 2, stream)
-        gbs = 8 * n / t / 1e9
-        ex[name] = {"ms": round(t * 1e3, 3), "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
-        log(f"[bench] {name}: {t * 1e3:.3f} ms, {gbs:.1f} GB/s")
-    del x
-    torch.cuda.empty_cache()
+    if want("dsum_2p28") or want("dnrm2_2p28"):
+        n = 1 << 28
+        x = wide_gpu(n, 3, dev)
+        for name, fn in (("dsum_2p28", lambda: L.reproblas_hip_rdsum_device(3, n, vp(x), 1, vp(out), sp)),
+                         ("dnrm2_2p28", lambda: L.reproblas_hip_rdnrm2_device(3, n, vp(x), 1, vp(out), sp))):
+            if not want(name):
+                continue
+            t = time_launches(fn, steps, 2, stream)
+            gbs = 8 * n / t / 1e9
+            ex[name] = {"ms": round(t * 1e3, 3), "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+            log(f"[bench] {name}: {t * 1e3:.3f} ms, {gbs:.1f} GB/s")
+        del x
+        torch.cuda.empty_cache()
 
-    # dgemv 16384^2 column-major, fold 3, alpha 1, beta 0.5
-    m = 16384
-    A = torch.randn(m * m, generator=g, device=dev, dtype=torch.float64)
-    xv = torch.randn(m, generator=g, device=dev, dtype=torch.float64)
-    yv = torch.randn(m, generator=g, device=dev, dtype=torch.float64)
-    for tr in ("N", "T"):
-        fn = lambda: L.reproblas_hip_rdgemv_device(3, b"C", tr.encode(), m, m, 1.0, vp(A), m, vp(xv), 1, 0.5, vp(yv),
-                                                   1, sp)
-        t = time_launches(fn, steps, 2, stream)
-        byts = 8 * (m * m + m + 2 * m)
-        ex[f"dgemv_{tr}_16384"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
-                                   "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
-        log(f"[bench] dgemv {tr}: {t * 1e3:.3f} ms, {byts / t / 1e9:.1f} GB/s")
-    del A
-    torch.cuda.empty_cache()
+    if want("dgemv"):
+        # dgemv 16384^2 column-major, fold 3, alpha 1, beta 0.5
+        m = 16384
+        A = torch.randn(m * m, generator=g, device=dev, dtype=torch.float64)
+        xv = torch.randn(m, generator=g, device=dev, dtype=torch.float64)
+        yv = torch.randn(m, generator=g, device=dev, dtype=torch.float64)
+        for tr in ("N", "T"):
+            fn = lambda: L.reproblas_hip_rdgemv_device(3, b"C", tr.encode(), m, m, 1.0, vp(A), m, vp(xv), 1, 0.5,
+                                                       vp(yv), 1, sp)
+            t = time_launches(fn, steps, 2, stream)
+            byts = 8 * (m * m + m + 2 * m)
+            ex[f"dgemv_{tr}_16384"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
+                                       "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
+            log(f"[bench] dgemv {tr}: {t * 1e3:.3f} ms, {byts / t / 1e9:.1f} GB/s")
+        ex["dgemv"] = {"N": ex["dgemv_N_16384"], "T": ex["dgemv_T_16384"]}
+        del A
+        torch.cuda.empty_cache()
 
-    # dgemm 4096^3 column-major NN, fold 3, alpha 1, beta 0
-    k = 4096
-    A = torch.randn(k * k, generator=g, device=dev, dtype=torch.float64)
-    B = torch.randn(k * k, generator=g, device=dev, dtype=torch.float64)
-    C = torch.zeros(k * k, device=dev, dtype=torch.float64)
-    fn = lambda: L.reproblas_hip_rdgemm_device(3, b"C", b"N", b"N", k, k, k, 1.0, vp(A), k, vp(B), k, 0.0, vp(C), k,
-                                               sp)
-    t = time_launches(fn, 2, 1, stream)
-    ops = 11.0 * k ** 3  # (4*fold-1) ops per product, the reference's model (bench_rdgemm.c:125-127)
-    ex["dgemm_NN_4096"] = {"ms": round(t * 1e3, 2), "GFLOP/s": round(ops / t / 1e9, 1),
-                           "valu_frac": round(ops / t / VALU_PEAK_OPS, 4),
-                           "products_per_s": round(k ** 3 / t, 1)}
-    log(f"[bench] dgemm: {t * 1e3:.2f} ms, {ops / t / 1e9:.1f} GFLOP/s (model ops), {ops / t / VALU_PEAK_OPS:.3f} of VALU")
+    if want("dgemm"):
+        # dgemm 4096^3 column-major NN, fold 3, alpha 1, beta 0
+        k = 4096
+        A = torch.randn(k * k, generator=g, device=dev, dtype=torch.float64)
+        B = torch.randn(k * k, generator=g, device=dev, dtype=torch.float64)
+        C = torch.zeros(k * k, device=dev, dtype=torch.float64)
+        fn = lambda: L.reproblas_hip_rdgemm_device(3, b"C", b"N", b"N", k, k, k, 1.0, vp(A), k, vp(B), k, 0.0, vp(C),
+                                                   k, sp)
+        t = time_launches(fn, 2, 1, stream)
+        ops = 11.0 * k ** 3  # (4*fold-1) ops per product, the reference's model (bench_rdgemm.c:125-127)
+        ex["dgemm"] = ex["dgemm_NN_4096"] = {"ms": round(t * 1e3, 2), "GFLOP/s": round(ops / t / 1e9, 1),
+                                             "valu_frac": round(ops / t / VALU_PEAK_OPS, 4),
+                                             "products_per_s": round(k ** 3 / t, 1)}
+        log(f"[bench] dgemm: {t * 1e3:.2f} ms, {ops / t / 1e9:.1f} GFLOP/s (model ops), "
+            f"{ops / t / VALU_PEAK_OPS:.3f} of VALU")
+        del A, B, C
+        torch.cuda.empty_cache()
+    if want("ragged_batch"):
+        ex["ragged_batch"] = ragged_batch(L, dev, stream, sp, vp)
+    if want("mlp"):
+        ex["mlp"] = ex["mlp_fwd_bwd"] = mlp(dev, stream, args)
+    if only is None:
+        ex.pop("dgemv", None)
+        ex.pop("dgemm", None)
+        ex.pop("mlp", None)
     return ex
+
+
+def ragged_batch(L, dev, stream, sp, vp):
+    """10,000 dot segments, lengths log-uniform in [16, 2^20] (CSR), wide-range
+    values, plus 512 column-major NN gemms with M, N, K uniform in
+    {32, 64, ..., 512}; fold 3. One graph launch of the batch plan against the
+    same work issued as per-call launches (each its own reproBLAS call)."""
+    import torch
+
+    from reproblas_amd.autotune import tune_gemm_batch
+    from reproblas_amd.batch import Batch
+
+    g = np.random.Generator(np.random.PCG64(7))
+    nseg = 10000
+    lens = np.exp(g.uniform(np.log(16), np.log(1 << 20), nseg)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(off[-1])
+    x = wide_gpu(total, 11, dev)
+    y = wide_gpu(total, 12, dev)
+    res = torch.zeros(nseg, dtype=torch.float64, device=dev)
+    res2 = torch.zeros(nseg, dtype=torch.float64, device=dev)
+    sizes = g.integers(1, 17, (512, 3)) * 32
+    gt = torch.Generator(device=dev)
+    gt.manual_seed(5)
+    triples = [(torch.randn(int(K), int(M), generator=gt, device=dev, dtype=torch.float64),
+                torch.randn(int(N), int(K), generator=gt, device=dev, dtype=torch.float64),
+                torch.zeros(int(N), int(M), device=dev, dtype=torch.float64)) for M, N, K in sizes]
+    log(f"[bench] ragged batch: {nseg} dot segments, {total} terms; 512 gemms, {int(np.prod(sizes, axis=1).sum())} products")
+    tuned = tune_gemm_batch("N", "N", triples, use_cache=False, reps=2)
+    b = Batch(3)
+    b.add_segments("dot", off, x, y, res)
+    b.add_gemms("N", "N", triples, tiling=tuned["tiling"])
+    t_graph = time_launches(lambda: b.launch(graph=True), 3, 1, stream)
+
+    xp, yp, rp = x.data_ptr(), y.data_ptr(), res2.data_ptr()
+
+    def per_call():
+        for s in range(nseg):
+            L.reproblas_hip_rddot_device(3, int(lens[s]), ctypes.c_void_p(xp + 8 * int(off[s])), 1,
+                                         ctypes.c_void_p(yp + 8 * int(off[s])), 1, ctypes.c_void_p(rp + 8 * s), sp)
+        for (M, N, K), (A, B, C) in zip(sizes, triples):
+            L.reproblas_hip_rdgemm_device(3, b"C", b"N", b"N", int(M), int(N), int(K), 1.0, vp(A), int(M), vp(B),
+                                          int(K), 0.0, vp(C), int(M), sp)
+
+    t_calls = time_launches(per_call, 1, 1, stream)
+    same = bool(torch.equal(res.view(torch.int64), res2.view(torch.int64)))
+    b.close()
+    log(f"[bench] ragged batch: graph {t_graph * 1e3:.2f} ms, per-call {t_calls * 1e3:.2f} ms, "
+        f"speedup {t_calls / t_graph:.2f}x, identical={same}")
+    return {"graph_ms": round(t_graph * 1e3, 3), "per_call_ms": round(t_calls * 1e3, 3),
+            "speedup": round(t_calls / t_graph, 2), "segments": nseg, "segment_terms": total, "gemms": 512,
+            "gemm_products": int(np.prod(sizes, axis=1).sum()), "tiling": tuned["tiling"],
+            "results_identical": same}
+
+
+def mlp(dev, stream, args):
+    """3-layer FP64 MLP 1024-4096-4096-1024 (tanh), batch 2048, MSE loss:
+    one forward + backward, every matmul binned (fold 3)."""
+    import torch
+
+    import reproblas_amd.autograd as ag
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    model = ag.ReproMLP((1024, 4096, 4096, 1024), generator=g, device=dev)
+    xb = torch.randn(2048, 1024, generator=g, device=dev, dtype=torch.float64)
+    tb = torch.randn(2048, 1024, generator=g, device=dev, dtype=torch.float64)
+    t = time_launches(lambda: ag.mlp_step(model, xb, tb), 2, 1, stream)
+    fwd = 2048 * (1024 * 4096 + 4096 * 4096 + 4096 * 1024)
+    products = fwd + 2048 * (1024 * 4096 + 4096 * 4096 + 4096 * 1024) + 2048 * (4096 * 4096 + 4096 * 1024)
+    log(f"[bench] mlp fwd+bwd: {t * 1e3:.1f} ms, {products / t / 1e12:.2f} T products/s")
+    return {"ms": round(t * 1e3, 2), "products_per_step": products,
+            "valu_frac": round(11.0 * products / t / VALU_PEAK_OPS, 4)}
 
 
 if __name__ == "__main__":

@@ -45,8 +45,19 @@ FLAGS = [
 ]
 
 
-def _compile(src):
+def _stale(src, obj):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps += [os.path.join(HERE, "..", "include", h) for h in os.listdir(os.path.join(HERE, "..", "include"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, obj):
+        return obj, ""
     cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -54,13 +65,16 @@ def _compile(src):
     return obj, r.stderr
 
 
-def build(jobs=None, verbose=False):
-    """Compile every HIP source for gfx950 and link the shared library."""
+def build(jobs=None, verbose=False, incremental=False):
+    """Compile every HIP source for gfx950 and link the shared library.
+
+    incremental=True skips sources whose object is newer than the source and
+    every header (for development); the default recompiles everything."""
     os.makedirs(OBJDIR, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 1)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(_compile, srcs))
+        results = list(ex.map(lambda f: _compile(f, incremental), srcs))
     if verbose:
         for _, err in results:
             if err.strip():
@@ -76,4 +90,4 @@ def build(jobs=None, verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="-i" in sys.argv))

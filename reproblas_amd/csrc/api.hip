@@ -23,7 +23,7 @@ constexpr double kNaN = __builtin_nan("");
 
 bool check_fold(int fold) {
   if (fold_supported(fold)) return true;
-  set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+  set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
   return false;
 }
 
@@ -56,7 +56,7 @@ bool unstage_vector(double* X, const double* d, int64_t n, int64_t inc, hipStrea
 double* pinned_scalars() {
   static thread_local double* p = nullptr;
   if (!p) {
-    if (!RB_HIP(hipHostMalloc((void**)&p, 64 * sizeof(double), hipHostMallocDefault))) p = nullptr;
+    if (!RB_HIP(hipHostMalloc((void**)&p, (16 + 4 * kMaxFold) * sizeof(double), hipHostMallocDefault))) p = nullptr;
   }
   return p;
 }
@@ -138,7 +138,7 @@ void reproblas_hip_clear_error(void) { clear_error(); }
 void reproblas_hip_set_stream(void* stream) { set_current_stream((hipStream_t)stream); }
 void* reproblas_hip_get_stream(void) { return (void*)current_stream(); }
 int reproblas_hip_min_fold(void) { return 2; }
-int reproblas_hip_max_fold(void) { return 8; }
+int reproblas_hip_max_fold(void) { return kMaxFold; }
 
 // ------------------------------------------------------------------ BLAS1 --
 // reference src/reproBLAS/rdsum.c:20-31

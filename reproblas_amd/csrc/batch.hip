@@ -103,7 +103,7 @@ extern "C" {
 reproblas_hip_batch* reproblas_hip_batch_create(int fold) {
   clear_error();
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return nullptr;
   }
   reproblas_hip_batch* b = new reproblas_hip_batch();

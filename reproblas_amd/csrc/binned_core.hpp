@@ -19,6 +19,9 @@
 #include <stdint.h>
 
 #define RB_HD __host__ __device__ __forceinline__
+// Loops over a binned value's slots: fully unrolled (registers) for the
+// small capacities, plain loops (private memory) for the general one.
+#define RB_UNROLL_CAP _Pragma("unroll (CAP <= 8 ? CAP : 1)")
 
 namespace rb {
 
@@ -84,15 +87,19 @@ RB_HD double dscale(double x) {
 // the fold in use (a constant after inlining for the fixed-fold kernels).
 template <int CAP>
 RB_HD double pick(const double (&a)[CAP], int k) {
-  double v = 0.0;
+  if constexpr (CAP > 8) {
+    return (k >= 0 && k < CAP) ? a[k] : 0.0;
+  } else {
+    double v = 0.0;
 #pragma unroll
-  for (int t = 0; t < CAP; ++t) v = (t == k) ? a[t] : v;
-  return v;
+    for (int t = 0; t < CAP; ++t) v = (t == k) ? a[t] : v;
+    return v;
+  }
 }
 
 template <int CAP>
 RB_HD void set_zero(int fold, double (&p)[CAP], double (&c)[CAP]) {
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) { p[j] = 0.0; c[j] = 0.0; }
 }
@@ -100,7 +107,7 @@ RB_HD void set_zero(int fold, double (&p)[CAP], double (&c)[CAP]) {
 // Fresh primaries of index I (reference src/binned/dmdupdate.c:36-41).
 template <int CAP>
 RB_HD void set_bins(int fold, int I, double (&p)[CAP], double (&c)[CAP]) {
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) { p[j] = bin(I + j); c[j] = 0.0; }
 }
@@ -112,7 +119,7 @@ template <int CAP>
 RB_HD void raise_index(int fold, int shift, int I, double (&p)[CAP], double (&c)[CAP]) {
   if (shift <= 0) return;
   double np[CAP], nc[CAP];
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) {
     if (j < fold) {
       if (j >= shift) {
@@ -124,7 +131,7 @@ RB_HD void raise_index(int fold, int shift, int I, double (&p)[CAP], double (&c)
       }
     }
   }
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) { p[j] = np[j]; c[j] = nc[j]; }
 }
@@ -153,7 +160,7 @@ RB_HD void deposit(int fold, bool top, double x, double (&p)[CAP]) {
     const double r = (m - q) * (kExpansion * 0.5);
     x = (x + r) + r;
   }
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) {
     if (j < fold - 1 && !(top && j == 0)) {
       const double m = p[j];
@@ -162,7 +169,7 @@ RB_HD void deposit(int fold, bool top, double x, double (&p)[CAP]) {
       x = x + (m - q);
     }
   }
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j == fold - 1) p[j] = p[j] + set_lsb(x);
 }
@@ -172,7 +179,7 @@ RB_HD void deposit(int fold, bool top, double x, double (&p)[CAP]) {
 template <int CAP>
 RB_HD void renorm(int fold, double (&p)[CAP], double (&c)[CAP]) {
   if (p[0] == 0.0 || nan_or_inf(p[0])) return;
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) {
     if (j < fold) {
       uint64_t u = bits(p[j]);
@@ -190,7 +197,7 @@ RB_HD void renorm(int fold, double (&p)[CAP], double (&c)[CAP]) {
 template <int CAP>
 RB_HD void add_same_index(int fold, int I, double (&py)[CAP], double (&cy)[CAP], const double (&px)[CAP],
                           const double (&cx)[CAP]) {
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) {
     if (j < fold) {
       py[j] = py[j] + (px[j] - bin(I + j));
@@ -205,7 +212,7 @@ template <int CAP>
 RB_HD void add(int fold, double (&py)[CAP], double (&cy)[CAP], const double (&px)[CAP], const double (&cx)[CAP]) {
   if (px[0] == 0.0) return;
   if (py[0] == 0.0) {
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) { py[j] = px[j]; cy[j] = cx[j]; }
     return;
@@ -219,7 +226,7 @@ RB_HD void add(int fold, double (&py)[CAP], double (&cy)[CAP], const double (&px
   const int shift = yi - xi;
   if (shift > 0) {
     double np[CAP], nc[CAP];
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
       if (j < fold) {
         if (j >= shift) {
@@ -231,11 +238,11 @@ RB_HD void add(int fold, double (&py)[CAP], double (&cy)[CAP], const double (&px
         }
       }
     }
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) { py[j] = np[j]; cy[j] = nc[j]; }
   } else {
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
       if (j < fold && j >= -shift) {
         py[j] += pick(px, j + shift) - bin(xi + j + shift);
@@ -288,7 +295,7 @@ RB_HD double to_double(int fold, const double (&p)[CAP], const double (&c)[CAP])
     return Y;
   }
   Y += c[0] * carry_unit(I);
-#pragma unroll
+RB_UNROLL_CAP
   for (int i = 1; i < CAP; ++i) {
     if (i < fold) {
       Y += c[i] * carry_unit(I + i);
@@ -307,12 +314,16 @@ RB_HD void rescale(int fold, double X, double scaleY, double (&p)[CAP], double (
   r *= r;
   for (int i = 0; i < fold; ++i) {
     double v = pick(p, i) / r;
+    if constexpr (CAP > 8) {
+      p[i] = v;
+    } else {
 #pragma unroll
-    for (int t = 0; t < CAP; ++t)
-      if (t == i) p[t] = v;
+      for (int t = 0; t < CAP; ++t)
+        if (t == i) p[t] = v;
+    }
     if (v == 0.0) {
       // dmdupdate(fold - i, 0.0, p + i, c + i): slots i.. become bins 51.. (empty)
-#pragma unroll
+RB_UNROLL_CAP
       for (int t = 0; t < CAP; ++t)
         if (t >= i && t < fold) { p[t] = bin(kMaxIndex + (t - i)); c[t] = 0.0; }
       return;
@@ -336,7 +347,7 @@ RB_HD double add_scaled(int fold, double sx, const double (&px)[CAP], const doub
     return sx;
   }
   double tp[CAP], tc[CAP];
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) { tp[j] = px[j]; tc[j] = cx[j]; }
   rescale(fold, sy, sx, tp, tc);
   add(fold, py, cy, tp, tc);

@@ -20,7 +20,7 @@ __global__ __launch_bounds__(256) void addv_kernel(int fold_rt, int64_t n, const
   const double* x = X + i * incX * 2 * fold;
   double* y = Y + i * incY * 2 * fold;
   double px[CAP], cx[CAP], py[CAP], cy[CAP];
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
       px[j] = x[j];
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void addv_kernel(int fold_rt, int64_t n, const
       cy[j] = y[fold + j];
     }
   add(fold, py, cy, px, cx);
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
       y[j] = py[j];
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void to_binned_kernel(int fold_rt, int64_t n, 
     renorm(fold, p, c);
   }
   double* y = Yb + i * 2 * fold;
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
       y[j] = p[j];
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void to_double_kernel(int fold_rt, int64_t n, 
   if (i >= n) return;
   const double* y = Yb + i * 2 * fold;
   double p[CAP], c[CAP];
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
       p[j] = y[j];
@@ -91,12 +91,17 @@ inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
     case 2: KERNEL<2, true><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;            \
     case 3: KERNEL<3, true><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;            \
     case 4: KERNEL<4, true><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;            \
-    default: KERNEL<8, false><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;          \
+    default:                                                                                  \
+      if (fold <= 8)                                                                          \
+        KERNEL<8, false><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__);                      \
+      else                                                                                    \
+        KERNEL<kMaxFold, false><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__);               \
+      break;                                                                                  \
   }
 
 bool dbdbaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int64_t incY, hipStream_t s) {
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return false;
   }
   if (n <= 0) return true;
@@ -106,7 +111,7 @@ bool dbdbaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int
 
 bool conv_to_binned(int fold, int64_t n, const double* y, int64_t incy, double beta, double* Yb, hipStream_t s) {
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return false;
   }
   if (n <= 0) return true;
@@ -116,7 +121,7 @@ bool conv_to_binned(int fold, int64_t n, const double* y, int64_t incy, double b
 
 bool conv_to_double(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s) {
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return false;
   }
   if (n <= 0) return true;

@@ -161,7 +161,7 @@ struct State {
 
 template <int CAP>
 __device__ __forceinline__ void store_state(double* dst, int fold, const State<CAP>& s) {
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
       dst[j] = s.p[j];
@@ -172,7 +172,7 @@ __device__ __forceinline__ void store_state(double* dst, int fold, const State<C
 
 template <int CAP>
 __device__ __forceinline__ void load_state(const double* src, int fold, State<CAP>& s) {
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
       s.p[j] = src[j];
@@ -188,7 +188,7 @@ __device__ __forceinline__ void merge(int fold, State<CAP>& y, const State<CAP>&
     if (nan_or_inf(x.p[0]) || nan_or_inf(y.p[0])) {
       // classes combine by addition; the scale no longer matters
       if (!nan_or_inf(y.p[0])) {
-#pragma unroll
+RB_UNROLL_CAP
         for (int j = 0; j < CAP; ++j) { y.p[j] = x.p[j]; y.c[j] = x.c[j]; }
         y.scl = x.scl;
       } else if (nan_or_inf(x.p[0])) {
@@ -208,7 +208,7 @@ __device__ __forceinline__ void wave_merge_general(int fold, State<CAP>& s) {
 #pragma unroll
   for (int m = 1; m <= 32; m <<= 1) {
     State<CAP> o;
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
       o.p[j] = shfl_xor_d(s.p[j], m);
       o.c[j] = shfl_xor_d(s.c[j], m);
@@ -335,7 +335,7 @@ __device__ __forceinline__ void wave_accumulate(const Span& a, int fold, int64_t
 #pragma unroll
   for (int m = 1; m <= 32; m <<= 1) {
     double op[CAP], oc[CAP];
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
       op[j] = shfl_xor_d(s.p[j], m);
       oc[j] = shfl_xor_d(s.c[j], m);
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
   for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
     State<CAP> o;
     const double* src = a.partials + (int64_t)b * (2 * fold + 1);
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) {
         o.p[j] = __builtin_nontemporal_load(src + j);
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
     // caller's value, renormalised first (the reference ends with a renorm)
     State<CAP> y;
     if (a.state_in) {
-#pragma unroll
+RB_UNROLL_CAP
       for (int j = 0; j < CAP; ++j)
         if (j < fold) {
           y.p[j] = a.state_in[j];
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
     double scale = y.scl;
     if (T == kSSq && nan_or_inf(y.p[0])) scale = 1.0;  // dscale(1.0), dmdssq.ccog tail
     if (a.state_out) {
-#pragma unroll
+RB_UNROLL_CAP
       for (int j = 0; j < CAP; ++j)
         if (j < fold) {
           a.state_out[j] = y.p[j];
@@ -569,13 +569,13 @@ bool launch_seg_term(Term term, const SegArgs& a, int grid, hipStream_t s) {
 
 }  // namespace
 
-bool fold_supported(int fold) { return fold >= 2 && fold <= 8; }
+bool fold_supported(int fold) { return fold >= 2 && fold <= kMaxFold; }
 
 bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
            const double* state_in, double scale_in, const double* scale_in_dev, double* state_out, double* scale_out,
            double* result, hipStream_t s) {
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return false;
   }
   const DeviceInfo* dev = device_info();
@@ -617,14 +617,16 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
     case 2: return launch_term<2, true>(term, mode, a, (int)blocks, s);
     case 3: return launch_term<3, true>(term, mode, a, (int)blocks, s);
     case 4: return launch_term<4, true>(term, mode, a, (int)blocks, s);
-    default: return launch_term<8, false>(term, mode, a, (int)blocks, s);
+    default:
+      if (fold <= 8) return launch_term<8, false>(term, mode, a, (int)blocks, s);
+      return launch_term<kMaxFold, false>(term, mode, a, (int)blocks, s);
   }
 }
 
 bool segments(int fold, Term term, const double* x, const double* y, const SegChunk* chunks, int64_t nchunks,
               double* partials, const int64_t* first_chunk, int64_t nseg, double* results, int grid, hipStream_t s) {
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return false;
   }
   SegArgs a;
@@ -642,7 +644,9 @@ bool segments(int fold, Term term, const double* x, const double* y, const SegCh
     case 2: return launch_seg_term<2, true>(term, a, grid, s);
     case 3: return launch_seg_term<3, true>(term, a, grid, s);
     case 4: return launch_seg_term<4, true>(term, a, grid, s);
-    default: return launch_seg_term<8, false>(term, a, grid, s);
+    default:
+      if (fold <= 8) return launch_seg_term<8, false>(term, a, grid, s);
+      return launch_seg_term<kMaxFold, false>(term, a, grid, s);
   }
 }
 

@@ -41,7 +41,7 @@ constexpr int KC = 32;
 template <int CAP>
 __device__ __forceinline__ void renorm_ic(int fold, double (&p)[CAP], int (&c)[CAP]) {
   if (p[0] == 0.0 || nan_or_inf(p[0])) return;
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) {
     if (j < fold) {
       // bits 51:50 of the mantissa live in bits 19:18 of the high word
@@ -58,13 +58,13 @@ __device__ __forceinline__ void raise_ic(int fold, int shift, int I, double (&p)
   if (shift <= 0) return;
   double np[CAP];
   int nc[CAP];
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) {
     if (j < fold) {
       if (j >= shift) {
         double v = 0.0;
         int w = 0;
-#pragma unroll
+RB_UNROLL_CAP
         for (int t = 0; t < CAP; ++t)
           if (t == j - shift) { v = p[t]; w = c[t]; }
         np[j] = v;
@@ -75,7 +75,7 @@ __device__ __forceinline__ void raise_ic(int fold, int shift, int I, double (&p)
       }
     }
   }
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) { p[j] = np[j]; c[j] = nc[j]; }
 }
@@ -127,7 +127,7 @@ __device__ __forceinline__ void gemm_tile(const GemmDesc& g, int64_t i0, int64_t
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j)
-#pragma unroll
+RB_UNROLL_CAP
       for (int f = 0; f < CAP; ++f) {
         p[i][j][f] = bin(kMaxIndex + f);
         c[i][j][f] = 0;
@@ -253,7 +253,7 @@ __device__ __forceinline__ void gemm_tile(const GemmDesc& g, int64_t i0, int64_t
       if (gi >= g.M || gj >= g.N) continue;
       renorm_ic(fold, p[i][j], c[i][j]);
       double ps[CAP], cs[CAP];
-#pragma unroll
+RB_UNROLL_CAP
       for (int f = 0; f < CAP; ++f) {
         ps[f] = p[i][j][f];
         cs[f] = (double)c[i][j][f];
@@ -274,13 +274,13 @@ __device__ __forceinline__ void gemm_tile(const GemmDesc& g, int64_t i0, int64_t
       if (g.Cb) {
         double* cbp = g.Cb + (gi * g.cbi + gj * g.cbj) * 2 * fold;
         double py[CAP], cy[CAP];
-#pragma unroll
+RB_UNROLL_CAP
         for (int f = 0; f < CAP; ++f)
           if (f < fold) { py[f] = cbp[f]; cy[f] = cbp[fold + f]; }
         renorm(fold, py, cy);
         add(fold, py, cy, ps, cs);
         renorm(fold, py, cy);
-#pragma unroll
+RB_UNROLL_CAP
         for (int f = 0; f < CAP; ++f)
           if (f < fold) { cbp[f] = py[f]; cbp[fold + f] = cy[f]; }
       } else {
@@ -390,7 +390,7 @@ bool launch_batch_fold(int tiling, bool ai, bool bk, const GemmDesc* d, const Ge
 bool gemm_common(const GemmDesc& g0, hipStream_t s) {
   const int fold = g0.fold;
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return false;
   }
   GemmDesc g = g0;
@@ -408,7 +408,9 @@ bool gemm_common(const GemmDesc& g0, hipStream_t s) {
     case 2: return launch_fold<2, true>(g_tiling, ai, bk, g, s);
     case 3: return launch_fold<3, true>(g_tiling, ai, bk, g, s);
     case 4: return launch_fold<4, true>(g_tiling, ai, bk, g, s);
-    default: return launch_fold<8, false>(g_tiling, ai, bk, g, s);
+    default:
+      if (fold <= 8) return launch_fold<8, false>(g_tiling, ai, bk, g, s);
+      return launch_fold<kMaxFold, false>(g_tiling, ai, bk, g, s);
   }
 }
 
@@ -448,7 +450,7 @@ void gemm_tile_shape(int tiling, int fold, int* bm, int* bn) {
 bool gemm_batch(int fold, int tiling, const GemmDesc* d_descs, const GemmDesc& desc0, int count, int64_t max_work,
                 const GemmTile* d_tiles, int64_t ntiles, int grid, hipStream_t s) {
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return false;
   }
   if (count <= 0 || ntiles <= 0) return true;
@@ -462,7 +464,9 @@ bool gemm_batch(int fold, int tiling, const GemmDesc* d_descs, const GemmDesc& d
     case 2: return launch_batch_fold<2, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
     case 3: return launch_batch_fold<3, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
     case 4: return launch_batch_fold<4, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
-    default: return launch_batch_fold<8, false>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
+    default:
+      if (fold <= 8) return launch_batch_fold<8, false>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
+      return launch_batch_fold<kMaxFold, false>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
   }
 }
 

@@ -81,13 +81,13 @@ __device__ __forceinline__ void finish_output(const GemvArgs& a, int fold, int64
   if (a.Yb) {
     double* yb = a.Yb + o * a.incYb * 2 * fold;
     double py[CAP], cy[CAP];
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) { py[j] = yb[j]; cy[j] = yb[fold + j]; }
     renorm(fold, py, cy);
     add(fold, py, cy, p, c);
     renorm(fold, py, cy);
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) { yb[j] = py[j]; yb[fold + j] = cy[j]; }
   } else {
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void rowdot_kernel(GemvArgs a) {
 #pragma unroll
   for (int s = 1; s <= 32; s <<= 1) {
     double op[CAP], oc[CAP];
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
       op[j] = shfl_xor_d(p[j], s);
       oc[j] = shfl_xor_d(c[j], s);
@@ -249,39 +249,45 @@ __global__ __launch_bounds__(kThreads) void colaxpy_kernel(GemvArgs a) {
   renorm(fold, p0, c0);
   renorm(fold, p1, c1);
 
-  // merge the four waves' values per output (general add), wave 0 writes
-  __shared__ double lds[kWaves - 1][128][2 * CAP];
-  if (wave > 0) {
-#pragma unroll
-    for (int j = 0; j < CAP; ++j)
-      if (j < fold) {
-        lds[wave - 1][2 * lane][j] = p0[j];
-        lds[wave - 1][2 * lane][CAP + j] = c0[j];
-        lds[wave - 1][2 * lane + 1][j] = p1[j];
-        lds[wave - 1][2 * lane + 1][CAP + j] = c1[j];
-      }
+  // merge the four waves' values per output: through LDS for the small
+  // capacities; the general capacity writes one partial per wave instead
+  double* part;
+  if constexpr (CAP <= 8) {
+    __shared__ double lds[kWaves - 1][128][2 * CAP];
+    if (wave > 0) {
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          lds[wave - 1][2 * lane][j] = p0[j];
+          lds[wave - 1][2 * lane][CAP + j] = c0[j];
+          lds[wave - 1][2 * lane + 1][j] = p1[j];
+          lds[wave - 1][2 * lane + 1][CAP + j] = c1[j];
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    for (int w = 0; w < kWaves - 1; ++w) {
+      double qp[CAP], qc[CAP];
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) { qp[j] = lds[w][2 * lane][j]; qc[j] = lds[w][2 * lane][CAP + j]; }
+      add(fold, p0, c0, qp, qc);
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) { qp[j] = lds[w][2 * lane + 1][j]; qc[j] = lds[w][2 * lane + 1][CAP + j]; }
+      add(fold, p1, c1, qp, qc);
+    }
+    part = a.partials + ((int64_t)blockIdx.y * a.nout) * 2 * fold;
+  } else {
+    part = a.partials + (((int64_t)blockIdx.y * kWaves + wave) * a.nout) * 2 * fold;
   }
-  __syncthreads();
-  if (wave != 0) return;
-  for (int w = 0; w < kWaves - 1; ++w) {
-    double qp[CAP], qc[CAP];
-#pragma unroll
-    for (int j = 0; j < CAP; ++j)
-      if (j < fold) { qp[j] = lds[w][2 * lane][j]; qc[j] = lds[w][2 * lane][CAP + j]; }
-    add(fold, p0, c0, qp, qc);
-#pragma unroll
-    for (int j = 0; j < CAP; ++j)
-      if (j < fold) { qp[j] = lds[w][2 * lane + 1][j]; qc[j] = lds[w][2 * lane + 1][CAP + j]; }
-    add(fold, p1, c1, qp, qc);
-  }
-  double* part = a.partials + ((int64_t)blockIdx.y * a.nout) * 2 * fold;
   if (in0) {
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) { part[o0 * 2 * fold + j] = p0[j]; part[o0 * 2 * fold + fold + j] = c0[j]; }
   }
   if (in1) {
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) { part[(o0 + 1) * 2 * fold + j] = p1[j]; part[(o0 + 1) * 2 * fold + fold + j] = c1[j]; }
   }
@@ -295,13 +301,14 @@ __global__ __launch_bounds__(kThreads) void colaxpy_finish(GemvArgs a) {
   if (o >= a.nout) return;
   double p[CAP], c[CAP];
   const double* src = a.partials + o * 2 * fold;
-#pragma unroll
+RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) { p[j] = src[j]; c[j] = src[fold + j]; }
-  for (int64_t s = 1; s < a.chunks; ++s) {
+  const int64_t records = (CAP <= 8) ? a.chunks : a.chunks * kWaves;
+  for (int64_t s = 1; s < records; ++s) {
     const double* q = a.partials + (s * a.nout + o) * 2 * fold;
     double qp[CAP], qc[CAP];
-#pragma unroll
+RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) { qp[j] = q[j]; qc[j] = q[fold + j]; }
     add(fold, p, c, qp, qc);
@@ -336,7 +343,8 @@ bool launch_gemv(bool rowdot, GemvArgs& a, hipStream_t s) {
   if (chunks > 65535) chunks = 65535;
   a.chunk = (a.len + chunks - 1) / chunks;
   a.chunks = (a.len + a.chunk - 1) / a.chunk;
-  a.partials = (double*)workspace(s, kSlotPartials, sizeof(double) * (size_t)a.chunks * a.nout * 2 * a.fold);
+  const int64_t records = (CAP <= 8) ? a.chunks : a.chunks * kWaves;
+  a.partials = (double*)workspace(s, kSlotPartials, sizeof(double) * (size_t)records * a.nout * 2 * a.fold);
   if (!a.partials) return false;
   const bool vec = ((uintptr_t)a.A % 16 == 0) && (a.lda % 2 == 0);
   dim3 grid((unsigned)rowblocks, (unsigned)a.chunks);
@@ -353,7 +361,7 @@ bool gemv_common(int fold, char order, char trans, int64_t M, int64_t N, double 
                  const double* x, int64_t incx, double* Yb, int64_t incYb, double* y, int64_t incy, double beta,
                  hipStream_t s) {
   if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 8]", fold);
+    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
     return false;
   }
   const bool rowmajor = (order == 'r' || order == 'R');
@@ -381,7 +389,9 @@ bool gemv_common(int fold, char order, char trans, int64_t M, int64_t N, double 
     case 2: return launch_gemv<2, true>(rowdot, a, s);
     case 3: return launch_gemv<3, true>(rowdot, a, s);
     case 4: return launch_gemv<4, true>(rowdot, a, s);
-    default: return launch_gemv<8, false>(rowdot, a, s);
+    default:
+      if (fold <= 8) return launch_gemv<8, false>(rowdot, a, s);
+      return launch_gemv<kMaxFold, false>(rowdot, a, s);
   }
 }
 

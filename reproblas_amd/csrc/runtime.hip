@@ -44,6 +44,7 @@ void set_error(int code, const char* fmt, ...) {
 void clear_error() {
   g_err = kOk;
   g_msg[0] = 0;
+  (void)hipGetLastError();  // drop a stale HIP error left by an earlier failed call
 }
 int last_error() { return g_err; }
 const char* last_error_message() { return g_msg; }

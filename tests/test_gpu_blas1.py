@@ -262,3 +262,25 @@ def test_full_size_dot_nrm2(gpu, fold):
     assert same(rb.rdnrm2(tx, fold), O.rdnrm2(fold, x))
     if fold == 3:
         assert same(rb.rdsum(tx, fold), O.rdsum(fold, x))
+
+
+@pytest.mark.parametrize("fold", [9, 13, 26, 52])
+def test_high_folds(gpu, fold):
+    """Folds above 8 run the general-capacity kernels (slots in private memory)."""
+    import torch
+
+    import reproblas_amd as rb
+
+    O = oracle()
+    for n in (1, 777, 70001):
+        x, y = dg.wide(n, 300 + n), dg.wide(n, 400 + n)
+        for op in OPS:
+            assert same(gpu_call(rb, op, fold, x, y), orc_call(op, fold, x, y)), (fold, n, op)
+    x = dg.special("mixed_range", 5000, 9)
+    assert same(rb.rdsum(x, fold), O.rdsum(fold, x))
+    Y, Ye = np.zeros(2 * fold), np.zeros(2 * fold)
+    rb.dbdsum(Y, x, fold)
+    O.dbdsum(fold, x, Ye)
+    assert np.array_equal(Y.view(np.uint64), Ye.view(np.uint64))
+    tx = torch.from_numpy(x).to(gpu)
+    assert same(rb.rdnrm2(tx, fold), O.rdnrm2(fold, x))

@@ -162,3 +162,28 @@ def test_gemv_full_size(gpu):
             ye = y0.copy()
             O.fn["rdgemv"](3, b"C", trans.encode(), n, n, 1.0, vp(A), n, vp(x), 1, beta, vp(ye), 1)
             assert eq_arrays(ty.cpu().numpy(), ye), (trans, beta)
+
+
+@pytest.mark.parametrize("fold", [5, 9, 20, 52])
+def test_high_fold_gemv_gemm(gpu, fold):
+    import reproblas_amd as rb
+
+    O = oracle()
+    M, N, K = 45, 37, 150
+    A = dg.wide(M * K, 1)
+    B = dg.wide(K * N, 2)
+    C0 = dg.normal(M * N, 3)
+    Ce, Cg = C0.copy(), C0.copy()
+    O.fn["rdgemm"](fold, b"C", b"N", b"T", M, N, K, 1.0, vp(A), M, vp(B), N, 0.5, vp(Ce), M)
+    rb.lib().reproBLAS_rdgemm(fold, b"C", b"N", b"T", M, N, K, 1.0, vp(A), M, vp(B), N, 0.5, vp(Cg), M)
+    rb._check()
+    assert eq_arrays(Cg, Ce)
+    x = dg.normal(K, 4)
+    for trans, lx, ly in (("N", K, M), ("T", M, K)):
+        y0 = dg.normal(ly, 5)
+        ye, yg = y0.copy(), y0.copy()
+        xv = x if lx == K else dg.normal(M, 6)
+        O.fn["rdgemv"](fold, b"C", trans.encode(), M, K, 2.0, vp(A), M, vp(xv), 1, 0.5, vp(ye), 1)
+        rb.lib().reproBLAS_rdgemv(fold, b"C", trans.encode(), M, K, 2.0, vp(A), M, vp(xv), 1, 0.5, vp(yg), 1)
+        rb._check()
+        assert eq_arrays(yg, ye), trans

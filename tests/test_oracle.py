@@ -68,7 +68,7 @@ def _vec(kind, n, seed):
 
 
 @pytest.mark.parametrize("kind", INPUTS)
-@pytest.mark.parametrize("fold", [2, 3, 4, 5, 8])
+@pytest.mark.parametrize("fold", [2, 3, 4, 5, 8, 13, 52])
 def test_blas1_matches_compiled_reference(kind, fold):
     O = oracle()
     for R in _refs():
