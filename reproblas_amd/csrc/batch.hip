@@ -66,6 +66,13 @@ struct reproblas_hip_batch {
 
 namespace {
 
+// Persistent workgroups per CU walking the chunk table of a segment group
+// (compile-time overridable for tuning builds).
+#ifndef RB_SEG_BLOCKS_PER_CU
+#define RB_SEG_BLOCKS_PER_CU 8
+#endif
+constexpr int kSegBlocksPerCU = RB_SEG_BLOCKS_PER_CU;
+
 bool launch_all(reproblas_hip_batch* b, hipStream_t s) {
   for (const SegGroup& g : b->segs)
     if (!segments(b->fold, g.term, g.x, g.y, g.d_chunks, g.nchunks, g.d_partials, g.d_first, g.nseg, g.results,
@@ -153,7 +160,7 @@ int reproblas_hip_batch_add_segments(reproblas_hip_batch* b, int op, int64_t nse
   const DeviceInfo* d = device_info();
   if (!d) return last_error();
   int64_t grid = (g.nchunks + 3) / 4;
-  grid = std::min<int64_t>(grid, (int64_t)d->cus * 8);
+  grid = std::min<int64_t>(grid, (int64_t)d->cus * kSegBlocksPerCU);
   g.grid = (int)std::max<int64_t>(grid, 1);
   b->segs.push_back(g);
   return kOk;

@@ -6,34 +6,23 @@
 // (dmdssq.ccog:60-101), merged the way binned_dbdbadd does
 // (src/binned/dmdmadd.c:27-74), converted by binned_ddmconv.
 //
-// Layout of the work. The vector is cut into wave tiles of 64*U terms. Each
-// wave walks tiles grid-stride. Every lane keeps its own fold-F primaries and
-// carries in VGPRs; the index (and, for nrm2, the scale) is common to the
-// wave. Per tile a lane takes the max |term| of its U terms; only when some
-// lane's max is no longer admissible at the wave's index does the wave take
-// the slow path (wave max, raise the index). Then every lane deposits its U
-// terms (the reference's deposit, element by element) and the wave
-// renormalises every 2048 deposits. At the end the 64 lanes merge by a
-// butterfly of exact same-index adds, the waves of a workgroup merge in LDS,
-// every workgroup writes one partial, and the last workgroup to finish (a
-// ticket counter) merges all partials, folds in the caller's binned value and
-// converts. HBM is read exactly once.
+// Layout of the work. The vector is cut into wave tiles of 64*U terms that
+// the waves walk grid-stride, each reducing its tiles into one binned value
+// (wave_sum.hpp). The waves of a workgroup merge in LDS, every workgroup
+// writes one partial, and the last workgroup to finish (a ticket counter)
+// merges all partials, folds in the caller's binned value and converts. HBM
+// is read exactly once.
 //
 // Why this equals the reference bit for bit: the slice each bin receives from
 // a term is independent of order and of the accumulator, the index ends at
 // dindex(max |term|) either way, and the merges are exact.
-//
-// Non-finite terms poison a lane's top primary (the "|1" of an infinity is a
-// NaN). A wave that ends poisoned re-reads its own tiles and replaces its top
-// primary by the class of its terms (NaN if any NaN or both infinities, else
-// the infinity); merges then combine classes by plain addition, as the
-// reference's infinity path does.
 #include <float.h>
 #include <limits.h>
 
 #include "binned_core.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
+#include "wave_sum.hpp"
 
 namespace rb {
 
@@ -59,303 +48,11 @@ struct Blas1Args {
   int fold;
 };
 
-__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
-
-__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-__device__ __forceinline__ double uniform_double(double v) {
-  const uint64_t u = bits(v);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
-  return from_bits(((uint64_t)hi << 32) | lo);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, shfl_xor_d(v, m));
-  return v;
-}
-
-// The term as the reference's infinity path adds it (dmdsum.ccog:68-72,
-// dmdasum and dmdssq use fabs, dmddot the product).
-template <int T>
-__device__ __forceinline__ double plain_term(double x, double y) {
-  if (T == kSum) return x;
-  if (T == kDot) return x * y;
-  return fabs(x);
-}
-
-typedef double d2 __attribute__((ext_vector_type(2)));
-
-// A strided vector range walked in wave tiles of 64*U terms.
-struct Span {
-  const double* x;
-  int64_t incx;
-  const double* y;  // second operand (dot) or null
-  int64_t incy;
-  int64_t n;
-  int64_t tiles;       // tiles covering n
-  int64_t full_tiles;  // tiles with no out-of-range element
-};
-
-template <int U>
-__device__ __forceinline__ Span make_span(const double* x, int64_t incx, const double* y, int64_t incy, int64_t n) {
-  Span sp;
-  sp.x = x;
-  sp.incx = incx;
-  sp.y = y;
-  sp.incy = incy;
-  sp.n = n;
-  sp.tiles = (n + 64 * U - 1) / (64 * U);
-  sp.full_tiles = n / (64 * U);
-  return sp;
-}
-
-// Loads the U raw terms of tile t for this lane. MODE 0: unit stride with
-// 16-byte vector loads (operands 16-byte aligned); MODE 1: unit stride, 8-byte
-// loads; MODE 2: arbitrary stride. Out-of-range terms read as 0, which every
-// deposit leaves unchanged.
-template <int T, int U, int MODE>
-__device__ __forceinline__ void load_tile(const Span& a, int64_t t, int lane, double (&vx)[U], double (&vy)[U]) {
-  const int64_t base = t * (64 * U);
-  const bool full = t < a.full_tiles;
-  if (MODE == 0 && full) {
-    // streaming 16-byte loads (non-temporal: every byte is read once)
-    const d2* xv = reinterpret_cast<const d2*>(a.x + base);
-#pragma unroll
-    for (int k = 0; k < U / 2; ++k) {
-      const d2 d = __builtin_nontemporal_load(xv + k * 64 + lane);
-      vx[2 * k] = d.x;
-      vx[2 * k + 1] = d.y;
-    }
-    if (T == kDot) {
-      const d2* yv = reinterpret_cast<const d2*>(a.y + base);
-#pragma unroll
-      for (int k = 0; k < U / 2; ++k) {
-        const d2 d = __builtin_nontemporal_load(yv + k * 64 + lane);
-        vy[2 * k] = d.x;
-        vy[2 * k + 1] = d.y;
-      }
-    }
-    return;
-  }
-#pragma unroll
-  for (int k = 0; k < U; ++k) {
-    const int64_t e = base + k * 64 + lane;
-    const bool in = full || e < a.n;
-    const int64_t ix = (MODE == 2) ? e * a.incx : e;
-    vx[k] = in ? a.x[ix] : 0.0;
-    if (T == kDot) {
-      const int64_t iy = (MODE == 2) ? e * a.incy : e;
-      vy[k] = in ? a.y[iy] : 0.0;
-    }
-  }
-}
-
-template <int CAP>
-struct State {
-  double p[CAP];
-  double c[CAP];
-  double scl;  // nrm2 only
-};
-
-template <int CAP>
-__device__ __forceinline__ void store_state(double* dst, int fold, const State<CAP>& s) {
-RB_UNROLL_CAP
-  for (int j = 0; j < CAP; ++j)
-    if (j < fold) {
-      dst[j] = s.p[j];
-      dst[fold + j] = s.c[j];
-    }
-  dst[2 * fold] = s.scl;
-}
-
-template <int CAP>
-__device__ __forceinline__ void load_state(const double* src, int fold, State<CAP>& s) {
-RB_UNROLL_CAP
-  for (int j = 0; j < CAP; ++j)
-    if (j < fold) {
-      s.p[j] = src[j];
-      s.c[j] = src[fold + j];
-    }
-  s.scl = src[2 * fold];
-}
-
-// General merge of two renormalised values (with scales for nrm2).
-template <int CAP, int T>
-__device__ __forceinline__ void merge(int fold, State<CAP>& y, const State<CAP>& x) {
-  if (T == kSSq) {
-    if (nan_or_inf(x.p[0]) || nan_or_inf(y.p[0])) {
-      // classes combine by addition; the scale no longer matters
-      if (!nan_or_inf(y.p[0])) {
-RB_UNROLL_CAP
-        for (int j = 0; j < CAP; ++j) { y.p[j] = x.p[j]; y.c[j] = x.c[j]; }
-        y.scl = x.scl;
-      } else if (nan_or_inf(x.p[0])) {
-        y.p[0] += x.p[0];
-      }
-      return;
-    }
-    y.scl = add_scaled(fold, x.scl, x.p, x.c, y.scl, y.p, y.c);
-  } else {
-    add(fold, y.p, y.c, x.p, x.c);
-  }
-}
-
-// Merge the 64 lanes of a wave (general add: lanes may differ in index).
-template <int CAP, int T>
-__device__ __forceinline__ void wave_merge_general(int fold, State<CAP>& s) {
-#pragma unroll
-  for (int m = 1; m <= 32; m <<= 1) {
-    State<CAP> o;
-RB_UNROLL_CAP
-    for (int j = 0; j < CAP; ++j) {
-      o.p[j] = shfl_xor_d(s.p[j], m);
-      o.c[j] = shfl_xor_d(s.c[j], m);
-    }
-    o.scl = shfl_xor_d(s.scl, m);
-    // lower lane of each pair keeps (y = lower, x = upper) so both lanes agree
-    merge<CAP, T>(fold, s, o);
-  }
-}
-
-// Class of the terms of the tiles this wave walked: NaN if any term is NaN or
-// both infinities occur, else the infinity that occurs. Terms as the
-// reference's infinity path forms them.
-template <int T, int U, int MODE>
-__device__ double wave_term_class(const Span& a, int64_t first, int64_t step, int lane) {
-  bool nan = false, pinf = false, ninf = false;
-  for (int64_t t = first; t < a.tiles; t += step) {
-    double vx[U], vy[U];
-    load_tile<T, U, MODE>(a, t, lane, vx, vy);
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const double v = plain_term<T>(vx[k], T == kDot ? vy[k] : 0.0);
-      nan |= (v != v);
-      pinf |= (v == __builtin_inf());
-      ninf |= (v == -__builtin_inf());
-    }
-  }
-  const bool any_nan = __any(nan), any_p = __any(pinf), any_n = __any(ninf);
-  if (any_nan || (any_p && any_n) || !(any_p || any_n)) return __builtin_nan("");
-  return any_p ? __builtin_inf() : -__builtin_inf();
-}
-
-// One wave reduces the tiles t0, t0 + tstep, ... of a span into `s` (every
-// lane ends with the wave's value; a poisoned value is replaced by the class
-// of the terms).
-template <int CAP, int T, int U, int MODE>
-__device__ __forceinline__ void wave_accumulate(const Span& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                                State<CAP>& s) {
-  // Every wave starts at index 51, the reference's representation of an
-  // empty sum (dmdupdate of 0.0 on a zero value), so every tile deposits.
-  int idx = kMaxIndex;
-  set_bins(fold, idx, s.p, s.c);
-  double lim = admit_limit(idx);
-  double scl = 0.0, inv = 0.0;  // nrm2: current scale of the wave and 1/scale
-  int dep = 0;
-
-  // the next tile's loads are issued before the current tile is deposited
-  double cx[U], cy[U];
-  if (t0 < a.tiles) load_tile<T, U, MODE>(a, t0, lane, cx, cy);
-  for (int64_t t = t0; t < a.tiles; t += tstep) {
-    double nx[U], ny[U];
-    const int64_t tn = t + tstep;
-    if (tn < a.tiles) load_tile<T, U, MODE>(a, tn, lane, nx, ny);
-    double v[U];
-    double m = 0.0;
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      if (T == kSum) v[k] = cx[k];
-      if (T == kASum) v[k] = fabs(cx[k]);
-      if (T == kDot) v[k] = cx[k] * cy[k];
-      if (T == kSSq) v[k] = cx[k];
-      m = fmax(m, fabs(v[k]));
-    }
-    bool need;
-    if (T == kSSq) {
-      const double ms = m * inv;
-      need = (dscale(m) > scl) || (ms * ms >= lim);
-    } else {
-      need = (m >= lim);
-    }
-    // the endurance renorm comes before the update as in the reference drivers
-    if (dep + U > kEndurance) {
-      renorm(fold, s.p, s.c);
-      dep = 0;
-    }
-    if (__any(need)) {
-      double M = uniform_double(wave_max(m));
-      if (T == kSSq) {
-        const double ns = dscale(M);
-        if (ns > scl) {
-          if (scl > 0.0) {
-            rescale(fold, ns, scl, s.p, s.c);
-            idx = uniform_int(dmindex(s.p[0]));
-          }
-          scl = ns;
-          inv = 1.0 / scl;
-        }
-        M = M / scl;  // reference dmdssq.ccog:98-99
-        M = M * M;
-      }
-      const int nI = dindex(M);
-      if (nI < idx) {
-        raise_index(fold, idx - nI, nI, s.p, s.c);
-        idx = nI;
-      }
-      lim = admit_limit(idx);
-    }
-    if (idx == 0) {
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        double tv = v[k];
-        if (T == kSSq) { tv = v[k] * inv; tv = tv * tv; }
-        deposit(fold, true, tv, s.p);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        double tv = v[k];
-        if (T == kSSq) { tv = v[k] * inv; tv = tv * tv; }
-        deposit(fold, false, tv, s.p);
-      }
-    }
-    dep += U;
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      cx[k] = nx[k];
-      if (T == kDot) cy[k] = ny[k];
-    }
-  }
-  renorm(fold, s.p, s.c);
-  s.scl = scl;
-
-  // Lanes share the index (and scale): exact same-index butterfly.
-#pragma unroll
-  for (int m = 1; m <= 32; m <<= 1) {
-    double op[CAP], oc[CAP];
-RB_UNROLL_CAP
-    for (int j = 0; j < CAP; ++j) {
-      op[j] = shfl_xor_d(s.p[j], m);
-      oc[j] = shfl_xor_d(s.c[j], m);
-    }
-    if (!nan_or_inf(s.p[0]) && !nan_or_inf(op[0])) {
-      add_same_index(fold, idx, s.p, s.c, op, oc);
-    } else {
-      s.p[0] = s.p[0] + op[0];  // keep the poison visible
-    }
-  }
-  if (nan_or_inf(uniform_double(s.p[0]))) {
-    s.p[0] = wave_term_class<T, U, MODE>(a, t0, tstep, lane);
-  }
-}
-
 template <int CAP, bool FIXED, int T, int U, int MODE>
 __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar)
   const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + wave;
   const int64_t nwaves = (int64_t)gridDim.x * kWaves;
   const Span sp = make_span<U>(a.x, a.incx, a.y, a.incy, a.n);
@@ -471,8 +168,28 @@ bool launch_mode(int mode, const Blas1Args& a, int blocks, hipStream_t s) {
 
 // Terms per lane per tile: 16 for the one-operand reductions, 8 for dot, so
 // every tile moves 8 KiB per wave.
-constexpr int kUOne = 16;
-constexpr int kUDot = 8;
+//
+// Grid: at most kBlocksPerCU workgroups (4 waves) per CU for unit stride. On
+// MI355X a 2^28-term sum streams at ~6.7 TB/s with one workgroup per CU and
+// drops to ~5.3 TB/s with eight: every wave keeps its next tile in flight, and
+// fewer concurrent streams keep a narrower window of HBM pages open.
+// (Compile-time overridable for tuning builds.)
+#ifndef RB_BLAS1_U_ONE
+#define RB_BLAS1_U_ONE 16
+#endif
+#ifndef RB_BLAS1_U_DOT
+#define RB_BLAS1_U_DOT 8
+#endif
+#ifndef RB_BLAS1_BLOCKS_PER_CU
+#define RB_BLAS1_BLOCKS_PER_CU 1
+#endif
+#ifndef RB_BLAS1_BLOCKS_PER_CU_STRIDED
+#define RB_BLAS1_BLOCKS_PER_CU_STRIDED 4
+#endif
+constexpr int kUOne = RB_BLAS1_U_ONE;
+constexpr int kUDot = RB_BLAS1_U_DOT;
+constexpr int kBlocksPerCU = RB_BLAS1_BLOCKS_PER_CU;
+constexpr int kBlocksPerCUStrided = RB_BLAS1_BLOCKS_PER_CU_STRIDED;
 
 template <int CAP, bool FIXED>
 bool launch_term(Term term, int mode, const Blas1Args& a, int blocks, hipStream_t s) {
@@ -506,7 +223,7 @@ template <int CAP, bool FIXED, int T, int U>
 __global__ __launch_bounds__(kThreads) void seg_chunk_kernel(SegArgs a) {
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
-  const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t nwaves = (int64_t)gridDim.x * kWaves;
   for (int64_t c = wave_gid; c < a.nchunks; c += nwaves) {
     const SegChunk ch = a.chunks[c];
@@ -596,15 +313,6 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
   a.result = result;
   a.fold = fold;
 
-  // grid: enough waves to keep every CU streaming, at most one wave per tile
-  int64_t blocks = (tiles + kWaves - 1) / kWaves;
-  const int64_t cap = (int64_t)dev->cus * 8;
-  if (blocks > cap) blocks = cap;
-  if (blocks < 1) blocks = 1;
-  a.partials = (double*)workspace(s, kSlotPartials, sizeof(double) * (size_t)blocks * (2 * fold + 1));
-  a.ticket = tickets(s, 1);
-  if (!a.partials || !a.ticket) return false;
-
   int mode;
   const bool unit = (incx == 1) && (term != kDot || incy == 1);
   if (!unit) {
@@ -613,6 +321,15 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
     const bool aligned = ((uintptr_t)x % 16 == 0) && (term != kDot || (uintptr_t)y % 16 == 0);
     mode = aligned ? 0 : 1;
   }
+
+  // grid: a few streaming waves per CU, at most one wave per tile
+  int64_t blocks = (tiles + kWaves - 1) / kWaves;
+  const int64_t cap = (int64_t)dev->cus * (mode == 2 ? kBlocksPerCUStrided : kBlocksPerCU);
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  a.partials = (double*)workspace(s, kSlotPartials, sizeof(double) * (size_t)blocks * (2 * fold + 1));
+  a.ticket = tickets(s, 1);
+  if (!a.partials || !a.ticket) return false;
   switch (fold) {
     case 2: return launch_term<2, true>(term, mode, a, (int)blocks, s);
     case 3: return launch_term<3, true>(term, mode, a, (int)blocks, s);

@@ -214,7 +214,9 @@ RB_UNROLL_CAP
 
     // ---- deposits: 1 mul + (4F-2) add/or per product
     if (!anytop) {
-#pragma unroll 1
+      // two k steps per trip: the bins then alternate between two register
+      // sets instead of being copied back at the end of every trip
+#pragma unroll 2
       for (int k = 0; k < KC; ++k) {
         double a[TM], b[TN];
 #pragma unroll

@@ -9,17 +9,22 @@
 //
 // Two memory patterns cover the four Order x Trans cases:
 //   row-dot  -- the terms of an output are contiguous (row-major N,
-//               column-major T): one wave per output streams its row with a
-//               wave-common index, like the BLAS1 dot.
+//               column-major T): fl(alpha*x) is first gathered into a
+//               contiguous vector; then a persistent grid of waves, one
+//               output per wave at a time, streams the rows as BLAS1 dots
+//               (wave_sum.hpp) and one lane folds in beta*y and converts.
 //   col-axpy -- the terms of an output are lda apart (column-major N,
 //               row-major T): each lane owns two adjacent outputs and walks
 //               the columns, so a wave reads 1 KiB contiguous per column with
 //               16-byte loads; every output keeps its own index. Column ranges
 //               are split across workgroups; a second kernel merges the
 //               per-range binned partials, folds in beta*y and converts.
+#include <algorithm>
+
 #include "binned_core.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
+#include "wave_sum.hpp"
 
 namespace rb {
 
@@ -27,14 +32,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-
-__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
-__device__ __forceinline__ double uniform_double(double v) {
-  const uint64_t u = bits(v);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
-  return from_bits(((uint64_t)hi << 32) | lo);
-}
 
 struct GemvArgs {
   int64_t nout, len;
@@ -100,77 +97,70 @@ RB_UNROLL_CAP
 }
 
 // ---------------------------------------------------------------- row-dot --
-template <int CAP, bool FIXED>
-__global__ __launch_bounds__(kThreads) void rowdot_kernel(GemvArgs a) {
-  constexpr int U = 8;
+// xs[k] = fl(alpha * x[k*incx]), the reference's scaled x (dbdgemv.ccog:86-95)
+__global__ __launch_bounds__(kThreads) void scale_x_kernel(GemvArgs a, double* xs) {
+  const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (k < a.len) xs[k] = scaled_x(a, k);
+}
+
+// Terms per lane per tile of the row-dot (the BLAS1 dot's shape: 8 KiB per
+// wave tile).
+constexpr int kURow = 8;
+
+__device__ __forceinline__ Span row_span(const GemvArgs& a, const double* xs, int64_t o) {
+  return make_span<kURow>(a.A + o * a.lda, 1, xs, 1, a.len);
+}
+
+// The wave's value of row o is complete: merge the lanes, convert, store.
+template <int CAP, int MODE>
+__device__ __forceinline__ void finish_row(const GemvArgs& a, const double* xs, int fold, int64_t o, int lane,
+                                           const WaveRun& w, State<CAP>& s) {
+  wave_finish<CAP, kDot, kURow, MODE>(row_span(a, xs, o), fold, 0, 1, lane, w, s);
+  if (lane == 0) finish_output(a, fold, o, s.p, s.c);
+}
+
+// Wave g of W owns rows g, g + W, ... and walks their tiles as one sequence,
+// so the next tile (the first of its next row, at a row's end) is in flight
+// while the current one is deposited; two register buffers take turns.
+template <int CAP, bool FIXED, int MODE>
+__global__ __launch_bounds__(kThreads) void rowdot_kernel(GemvArgs a, const double* xs) {
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
-  const int64_t o = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t nw = (int64_t)gridDim.x * kWaves;
+  int64_t o = (int64_t)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (o >= a.nout) return;  // whole wave leaves together
-  const double* row = a.A + o * a.lda;
-  double p[CAP], c[CAP];
-  int idx = kMaxIndex;
-  set_bins(fold, idx, p, c);
-  double lim = admit_limit(idx);
-  int dep = 0;
-  const int64_t tiles = (a.len + 64 * U - 1) / (64 * U);
-  for (int64_t t = 0; t < tiles; ++t) {
-    double v[U];
-    double m = 0.0;
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int64_t e = t * (64 * U) + k * 64 + lane;
-      v[k] = (e < a.len) ? row[e] * scaled_x(a, e) : 0.0;
-      m = fmax(m, fabs(v[k]));
+  const int64_t tiles = (a.len + 64 * kURow - 1) / (64 * kURow);
+  State<CAP> s;
+  WaveRun w;
+  wave_start(fold, w, s);
+  double ax[kURow], ay[kURow], bx[kURow], by[kURow];
+  int64_t t = 0;
+  load_tile<kDot, kURow, MODE>(row_span(a, xs, o), t, lane, ax, ay);
+  while (true) {
+    int64_t on = o, tn = t + 1;
+    if (tn == tiles) { on = o + nw; tn = 0; }
+    if (on < a.nout) load_tile<kDot, kURow, MODE>(row_span(a, xs, on), tn, lane, bx, by);
+    deposit_tile<CAP, kDot, kURow>(fold, ax, ay, w, s);
+    if (tn == 0) {
+      finish_row<CAP, MODE>(a, xs, fold, o, lane, w, s);
+      wave_start(fold, w, s);
     }
-    if (dep + U > kEndurance) {
-      renorm(fold, p, c);
-      dep = 0;
+    if (on >= a.nout) break;
+    o = on;
+    t = tn;
+    on = o;
+    tn = t + 1;
+    if (tn == tiles) { on = o + nw; tn = 0; }
+    if (on < a.nout) load_tile<kDot, kURow, MODE>(row_span(a, xs, on), tn, lane, ax, ay);
+    deposit_tile<CAP, kDot, kURow>(fold, bx, by, w, s);
+    if (tn == 0) {
+      finish_row<CAP, MODE>(a, xs, fold, o, lane, w, s);
+      wave_start(fold, w, s);
     }
-    if (__any(m >= lim)) {
-      double M = m;
-#pragma unroll
-      for (int s = 32; s >= 1; s >>= 1) M = fmax(M, shfl_xor_d(M, s));
-      const int nI = dindex(uniform_double(M));
-      if (nI < idx) {
-        raise_index(fold, idx - nI, nI, p, c);
-        idx = nI;
-      }
-      lim = admit_limit(idx);
-    }
-    const bool top = (idx == 0);
-#pragma unroll
-    for (int k = 0; k < U; ++k) deposit(fold, top, v[k], p);
-    dep += U;
+    if (on >= a.nout) break;
+    o = on;
+    t = tn;
   }
-  renorm(fold, p, c);
-#pragma unroll
-  for (int s = 1; s <= 32; s <<= 1) {
-    double op[CAP], oc[CAP];
-RB_UNROLL_CAP
-    for (int j = 0; j < CAP; ++j) {
-      op[j] = shfl_xor_d(p[j], s);
-      oc[j] = shfl_xor_d(c[j], s);
-    }
-    if (!nan_or_inf(p[0]) && !nan_or_inf(op[0])) {
-      add_same_index(fold, idx, p, c, op, oc);
-    } else {
-      p[0] = p[0] + op[0];
-    }
-  }
-  if (nan_or_inf(uniform_double(p[0]))) {
-    // class of the terms, as the reference's infinity path would add them
-    bool nan = false, pinf = false, ninf = false;
-    for (int64_t e = lane; e < a.len; e += 64) {
-      const double t = row[e] * scaled_x(a, e);
-      nan |= (t != t);
-      pinf |= (t == __builtin_inf());
-      ninf |= (t == -__builtin_inf());
-    }
-    const bool an = __any(nan), ap = __any(pinf), aneg = __any(ninf);
-    p[0] = (an || (ap && aneg) || !(ap || aneg)) ? __builtin_nan("") : (ap ? __builtin_inf() : -__builtin_inf());
-  }
-  if (lane == 0) finish_output(a, fold, o, p, c);
 }
 
 // --------------------------------------------------------------- col-axpy --
@@ -181,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void colaxpy_kernel(GemvArgs a) {
   constexpr int U = 4;  // columns per step
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar)
   const int64_t o0 = (int64_t)blockIdx.x * 128 + 2 * lane;
   const int64_t kbeg = (int64_t)blockIdx.y * a.chunk;
   int64_t kend = kbeg + a.chunk;
@@ -326,15 +316,29 @@ RB_UNROLL_CAP
   finish_output(a, fold, o, p, c);
 }
 
+// Persistent row-dot workgroups per CU (compile-time overridable for tuning
+// builds).
+#ifndef RB_GEMV_ROW_BLOCKS_PER_CU
+#define RB_GEMV_ROW_BLOCKS_PER_CU 4
+#endif
+constexpr int kRowBlocksPerCU = RB_GEMV_ROW_BLOCKS_PER_CU;
+
 template <int CAP, bool FIXED>
 bool launch_gemv(bool rowdot, GemvArgs& a, hipStream_t s) {
-  if (rowdot) {
-    const int64_t blocks = (a.nout + kWaves - 1) / kWaves;
-    rowdot_kernel<CAP, FIXED><<<(unsigned)blocks, kThreads, 0, s>>>(a);
-    return RB_HIP(hipGetLastError());
-  }
   const DeviceInfo* d = device_info();
   if (!d) return false;
+  if (rowdot) {
+    double* xs = (double*)workspace(s, kSlotGemvX, sizeof(double) * (size_t)a.len);
+    if (!xs) return false;
+    scale_x_kernel<<<(unsigned)((a.len + kThreads - 1) / kThreads), kThreads, 0, s>>>(a, xs);
+    const int64_t blocks = std::min<int64_t>((a.nout + kWaves - 1) / kWaves, (int64_t)d->cus * kRowBlocksPerCU);
+    const bool vec = ((uintptr_t)a.A % 16 == 0) && (a.lda % 2 == 0);
+    if (vec)
+      rowdot_kernel<CAP, FIXED, 0><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
+    else
+      rowdot_kernel<CAP, FIXED, 1><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
+    return RB_HIP(hipGetLastError());
+  }
   const int64_t rowblocks = (a.nout + 127) / 128;
   int64_t chunks = (4 * (int64_t)d->cus + rowblocks - 1) / rowblocks;
   const int64_t max_chunks = (a.len + 255) / 256;  // keep >= 256 columns per chunk

@@ -64,7 +64,8 @@ enum WorkspaceSlot : int {
   kSlotBinnedOut = 5,
   kSlotBatch = 6,
   kSlotGemmMax = 7,  // per-chunk row/column maxima of the gemm operands
-  kSlotCount = 8,
+  kSlotGemvX = 8,    // fl(alpha * x) gathered contiguously for the row-dot gemv
+  kSlotCount = 9,
 };
 
 }  // namespace rb

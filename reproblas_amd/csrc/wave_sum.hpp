@@ -1,0 +1,360 @@
+// wave_sum.hpp -- one wave's binned reduction of a strided vector range,
+// shared by the BLAS1 kernels, the ragged segment kernels and the row-dot
+// gemv kernel.
+//
+// Reference path: the per-block loops of binnedBLAS_dmdsum
+// (src/binnedBLAS/dmdsum.ccog:57-89), dmdasum (dmdasum.ccog:57-85), dmddot
+// (dmddot.ccog:59-88) and dmdssq (dmdssq.ccog:60-101), and the merge of
+// binned_dbdbadd (src/binned/dmdmadd.c:27-74).
+//
+// A wave walks tiles of 64*U terms (t0, t0 + tstep, ...). Every lane keeps its
+// own fold-F primaries and carries in VGPRs; the index (and, for nrm2, the
+// scale) is common to the wave. Per tile a lane takes the max |term| of its U
+// terms; only when some lane's max is no longer admissible at the wave's
+// index does the wave take the slow path (wave max, raise the index). Then
+// every lane deposits its U terms (the reference's deposit, element by
+// element) and the wave renormalises every 2048 deposits. The 64 lanes merge
+// by a butterfly of exact same-index adds.
+//
+// Non-finite terms poison a lane's top primary (the "|1" of an infinity is a
+// NaN). A wave that ends poisoned re-reads its own tiles and replaces its top
+// primary by the class of its terms (NaN if any NaN or both infinities, else
+// the infinity); merges then combine classes by plain addition, as the
+// reference's infinity path does.
+//
+// Included into an anonymous namespace of each kernel source.
+#pragma once
+
+#include "binned_core.hpp"
+#include "kernels.hpp"
+
+namespace rb {
+namespace {
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
+
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double uniform_double(double v) {
+  const uint64_t u = bits(v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return from_bits(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, shfl_xor_d(v, m));
+  return v;
+}
+
+// The term as the reference's infinity path adds it (dmdsum.ccog:68-72,
+// dmdasum and dmdssq use fabs, dmddot the product).
+template <int T>
+__device__ __forceinline__ double plain_term(double x, double y) {
+  if (T == kSum) return x;
+  if (T == kDot) return x * y;
+  return fabs(x);
+}
+
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// A strided vector range walked in wave tiles of 64*U terms.
+struct Span {
+  const double* x;
+  int64_t incx;
+  const double* y;  // second operand (dot) or null
+  int64_t incy;
+  int64_t n;
+  int64_t tiles;       // tiles covering n
+  int64_t full_tiles;  // tiles with no out-of-range element
+};
+
+template <int U>
+__device__ __forceinline__ Span make_span(const double* x, int64_t incx, const double* y, int64_t incy, int64_t n) {
+  Span sp;
+  sp.x = x;
+  sp.incx = incx;
+  sp.y = y;
+  sp.incy = incy;
+  sp.n = n;
+  sp.tiles = (n + 64 * U - 1) / (64 * U);
+  sp.full_tiles = n / (64 * U);
+  return sp;
+}
+
+// Loads the U raw terms of tile t for this lane. MODE 0: unit stride with
+// 16-byte vector loads (operands 16-byte aligned); MODE 1: unit stride, 8-byte
+// loads; MODE 2: arbitrary stride. Out-of-range terms read as 0, which every
+// deposit leaves unchanged.
+template <int T, int U, int MODE>
+__device__ __forceinline__ void load_tile(const Span& a, int64_t t, int lane, double (&vx)[U], double (&vy)[U]) {
+  const int64_t base = t * (64 * U);
+  const bool full = t < a.full_tiles;
+  if (MODE == 0 && full) {
+    // streaming 16-byte loads (non-temporal: every byte is read once)
+    const d2* xv = reinterpret_cast<const d2*>(a.x + base);
+#pragma unroll
+    for (int k = 0; k < U / 2; ++k) {
+      const d2 d = __builtin_nontemporal_load(xv + k * 64 + lane);
+      vx[2 * k] = d.x;
+      vx[2 * k + 1] = d.y;
+    }
+    if (T == kDot) {
+      const d2* yv = reinterpret_cast<const d2*>(a.y + base);
+#pragma unroll
+      for (int k = 0; k < U / 2; ++k) {
+        const d2 d = __builtin_nontemporal_load(yv + k * 64 + lane);
+        vy[2 * k] = d.x;
+        vy[2 * k + 1] = d.y;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const int64_t e = base + k * 64 + lane;
+    const bool in = full || e < a.n;
+    const int64_t ix = (MODE == 2) ? e * a.incx : e;
+    vx[k] = in ? a.x[ix] : 0.0;
+    if (T == kDot) {
+      const int64_t iy = (MODE == 2) ? e * a.incy : e;
+      vy[k] = in ? a.y[iy] : 0.0;
+    }
+  }
+}
+
+template <int CAP>
+struct State {
+  double p[CAP];
+  double c[CAP];
+  double scl;  // nrm2 only
+};
+
+template <int CAP>
+__device__ __forceinline__ void store_state(double* dst, int fold, const State<CAP>& s) {
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) {
+      dst[j] = s.p[j];
+      dst[fold + j] = s.c[j];
+    }
+  dst[2 * fold] = s.scl;
+}
+
+template <int CAP>
+__device__ __forceinline__ void load_state(const double* src, int fold, State<CAP>& s) {
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) {
+      s.p[j] = src[j];
+      s.c[j] = src[fold + j];
+    }
+  s.scl = src[2 * fold];
+}
+
+// General merge of two renormalised values (with scales for nrm2).
+template <int CAP, int T>
+__device__ __forceinline__ void merge(int fold, State<CAP>& y, const State<CAP>& x) {
+  if (T == kSSq) {
+    if (nan_or_inf(x.p[0]) || nan_or_inf(y.p[0])) {
+      // classes combine by addition; the scale no longer matters
+      if (!nan_or_inf(y.p[0])) {
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j) { y.p[j] = x.p[j]; y.c[j] = x.c[j]; }
+        y.scl = x.scl;
+      } else if (nan_or_inf(x.p[0])) {
+        y.p[0] += x.p[0];
+      }
+      return;
+    }
+    y.scl = add_scaled(fold, x.scl, x.p, x.c, y.scl, y.p, y.c);
+  } else {
+    add(fold, y.p, y.c, x.p, x.c);
+  }
+}
+
+// Merge the 64 lanes of a wave (general add: lanes may differ in index).
+template <int CAP, int T>
+__device__ __forceinline__ void wave_merge_general(int fold, State<CAP>& s) {
+#pragma unroll
+  for (int m = 1; m <= 32; m <<= 1) {
+    State<CAP> o;
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j) {
+      o.p[j] = shfl_xor_d(s.p[j], m);
+      o.c[j] = shfl_xor_d(s.c[j], m);
+    }
+    o.scl = shfl_xor_d(s.scl, m);
+    // lower lane of each pair keeps (y = lower, x = upper) so both lanes agree
+    merge<CAP, T>(fold, s, o);
+  }
+}
+
+// Class of the terms of the tiles this wave walked: NaN if any term is NaN or
+// both infinities occur, else the infinity that occurs. Terms as the
+// reference's infinity path forms them.
+template <int T, int U, int MODE>
+__device__ double wave_term_class(const Span& a, int64_t first, int64_t step, int lane) {
+  bool nan = false, pinf = false, ninf = false;
+  for (int64_t t = first; t < a.tiles; t += step) {
+    double vx[U], vy[U];
+    load_tile<T, U, MODE>(a, t, lane, vx, vy);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const double v = plain_term<T>(vx[k], T == kDot ? vy[k] : 0.0);
+      nan |= (v != v);
+      pinf |= (v == __builtin_inf());
+      ninf |= (v == -__builtin_inf());
+    }
+  }
+  const bool any_nan = __any(nan), any_p = __any(pinf), any_n = __any(ninf);
+  if (any_nan || (any_p && any_n) || !(any_p || any_n)) return __builtin_nan("");
+  return any_p ? __builtin_inf() : -__builtin_inf();
+}
+
+// Per-wave running state of wave_accumulate.
+struct WaveRun {
+  int idx;     // common index of the wave's bins
+  double lim;  // admit_limit(idx)
+  double scl;  // nrm2: current scale of the wave
+  double inv;  // nrm2: 1 / scl
+  int dep;     // deposits since the last renormalisation
+};
+
+// Deposits one tile of raw terms (cx, cy) into the lane's bins.
+template <int CAP, int T, int U>
+__device__ __forceinline__ void deposit_tile(int fold, const double (&cx)[U], const double (&cy)[U], WaveRun& w,
+                                             State<CAP>& s) {
+  double v[U];
+  double m = 0.0;
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    if (T == kSum) v[k] = cx[k];
+    if (T == kASum) v[k] = fabs(cx[k]);
+    if (T == kDot) v[k] = cx[k] * cy[k];
+    if (T == kSSq) v[k] = cx[k];
+    m = fmax(m, fabs(v[k]));
+  }
+  bool need;
+  if (T == kSSq) {
+    const double ms = m * w.inv;
+    need = (dscale(m) > w.scl) || (ms * ms >= w.lim);
+  } else {
+    need = (m >= w.lim);
+  }
+  // the endurance renorm comes before the update as in the reference drivers
+  if (w.dep + U > kEndurance) {
+    renorm(fold, s.p, s.c);
+    w.dep = 0;
+  }
+  if (__any(need)) {
+    double M = uniform_double(wave_max(m));
+    if (T == kSSq) {
+      const double ns = dscale(M);
+      if (ns > w.scl) {
+        if (w.scl > 0.0) {
+          rescale(fold, ns, w.scl, s.p, s.c);
+          w.idx = uniform_int(dmindex(s.p[0]));
+        }
+        w.scl = ns;
+        w.inv = 1.0 / w.scl;
+      }
+      M = M / w.scl;  // reference dmdssq.ccog:98-99
+      M = M * M;
+    }
+    const int nI = dindex(M);
+    if (nI < w.idx) {
+      raise_index(fold, w.idx - nI, nI, s.p, s.c);
+      w.idx = nI;
+    }
+    w.lim = admit_limit(w.idx);
+  }
+  if (w.idx == 0) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      double tv = v[k];
+      if (T == kSSq) { tv = v[k] * w.inv; tv = tv * tv; }
+      deposit(fold, true, tv, s.p);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      double tv = v[k];
+      if (T == kSSq) { tv = v[k] * w.inv; tv = tv * tv; }
+      deposit(fold, false, tv, s.p);
+    }
+  }
+  w.dep += U;
+}
+
+// Fresh wave state: every wave starts at index 51, the reference's
+// representation of an empty sum (dmdupdate of 0.0 on a zero value), so every
+// tile deposits.
+template <int CAP>
+__device__ __forceinline__ void wave_start(int fold, WaveRun& w, State<CAP>& s) {
+  w.idx = kMaxIndex;
+  set_bins(fold, w.idx, s.p, s.c);
+  w.lim = admit_limit(w.idx);
+  w.scl = 0.0;
+  w.inv = 0.0;
+  w.dep = 0;
+}
+
+// Ends a wave's reduction of the tiles t0, t0 + tstep, ... of `a`: renorm,
+// exact same-index butterfly over the lanes (they share the index and scale),
+// and, if the value is poisoned, the class of the terms.
+template <int CAP, int T, int U, int MODE>
+__device__ __forceinline__ void wave_finish(const Span& a, int fold, int64_t t0, int64_t tstep, int lane,
+                                            const WaveRun& w, State<CAP>& s) {
+  renorm(fold, s.p, s.c);
+  s.scl = w.scl;
+#pragma unroll
+  for (int m = 1; m <= 32; m <<= 1) {
+    double op[CAP], oc[CAP];
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j) {
+      op[j] = shfl_xor_d(s.p[j], m);
+      oc[j] = shfl_xor_d(s.c[j], m);
+    }
+    if (!nan_or_inf(s.p[0]) && !nan_or_inf(op[0])) {
+      add_same_index(fold, w.idx, s.p, s.c, op, oc);
+    } else {
+      s.p[0] = s.p[0] + op[0];  // keep the poison visible
+    }
+  }
+  if (nan_or_inf(uniform_double(s.p[0]))) {
+    s.p[0] = wave_term_class<T, U, MODE>(a, t0, tstep, lane);
+  }
+}
+
+// One wave reduces the tiles t0, t0 + tstep, ... of a span into `s` (every
+// lane ends with the wave's value; a poisoned value is replaced by the class
+// of the terms).
+template <int CAP, int T, int U, int MODE>
+__device__ __forceinline__ void wave_accumulate(const Span& a, int fold, int64_t t0, int64_t tstep, int lane,
+                                                State<CAP>& s) {
+  WaveRun w;
+  wave_start(fold, w, s);
+
+  // Two tile buffers in turn: the loads of the next tile are in flight while
+  // the current one is deposited, and no buffer is ever copied.
+  double ax[U], ay[U], bx[U], by[U];
+  int64_t t = t0;
+  if (t < a.tiles) load_tile<T, U, MODE>(a, t, lane, ax, ay);
+  while (t < a.tiles) {
+    int64_t tn = t + tstep;
+    if (tn < a.tiles) load_tile<T, U, MODE>(a, tn, lane, bx, by);
+    deposit_tile<CAP, T, U>(fold, ax, ay, w, s);
+    if (tn >= a.tiles) break;
+    t = tn;
+    tn = t + tstep;
+    if (tn < a.tiles) load_tile<T, U, MODE>(a, tn, lane, ax, ay);
+    deposit_tile<CAP, T, U>(fold, bx, by, w, s);
+    t = tn;
+  }
+  wave_finish<CAP, T, U, MODE>(a, fold, t0, tstep, lane, w, s);
+}
+
+}  // namespace
+}  // namespace rb
