@@ -141,15 +141,15 @@ def main():
     log(f"[bench] ddot {t * 1e3:.3f} ms/call, {value:.1f} GB/s")
 
     # bit-exactness spot check on a sample against the CPU reference
-    ns = 1 << 24
-    xs = x[:ns].cpu().numpy()
-    ys = y[:ns].cpu().numpy()
-    L.reproblas_hip_rddot_device(3, ns, vp(x), 1, vp(y), 1, vp(out[1:]), sp)
-    torch.cuda.synchronize()
-    gpu_sample = float(out[1].item())
     cpu = None
-    ref_sample = None
+    ref_sample = gpu_sample = None
     if not args.no_cpu:
+        ns = 1 << 24
+        xs = x[:ns].cpu().numpy()
+        ys = y[:ns].cpu().numpy()
+        L.reproblas_hip_rddot_device(3, ns, vp(x), 1, vp(y), 1, vp(out[1:]), sp)
+        torch.cuda.synchronize()
+        gpu_sample = float(out[1].item())
         log("[bench] CPU baseline")
         cpu, ref_sample = cpu_baseline(xs, ys)
     traffic = None

@@ -5,14 +5,17 @@
 // group (many small gemms of one fold and layout). Planning happens once on
 // the host: segments are cut into chunks, gemms into output tiles sorted by K
 // (heaviest first); the tables live in device memory owned by the plan.
-// Launching runs, per group, a persistent grid that walks its table plus a
-// small finishing kernel; with graph launch the whole plan is captured once
-// into a hipGraph and replayed. No kernel of a plan writes through memset or
+// Launching runs, per segment group, a persistent grid that walks its chunk
+// table plus a small finishing kernel, and, for the gemm group, one workgroup
+// per output tile on a forked side stream (overlapping the HBM-bound segments
+// with the VALU-bound gemms); with graph launch the whole plan is captured
+// once into a hipGraph and replayed. No kernel of a plan writes through memset or
 // atomics, so replays need no resets.
 //
 // None of this has a counterpart in the reference, which offers one call per
 // reduction; the bits of every result equal that call's (one binned value per
 // segment or output, merged exactly).
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -60,6 +63,9 @@ struct reproblas_hip_batch {
   std::vector<SegGroup> segs;
   std::vector<GemmGroup> gemms;
   hipStream_t capture = nullptr;
+  hipStream_t side = nullptr;  // gemm branch of a plan that also has segments
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -67,21 +73,43 @@ struct reproblas_hip_batch {
 namespace {
 
 // Persistent workgroups per CU walking the chunk table of a segment group
-// (compile-time overridable for tuning builds).
+// (compile-time overridable for tuning builds). Two leave VGPR room on every
+// SIMD for gemm tiles running beside them.
 #ifndef RB_SEG_BLOCKS_PER_CU
-#define RB_SEG_BLOCKS_PER_CU 8
+#define RB_SEG_BLOCKS_PER_CU 2
 #endif
 constexpr int kSegBlocksPerCU = RB_SEG_BLOCKS_PER_CU;
 
+// Segment groups run on `s`. When the plan also has gemms, they run on a side
+// stream forked from `s` and joined back into it: the segments are HBM-bound
+// and the gemm tiles VALU-bound, so the two overlap on the same CUs. Under
+// graph capture the fork and join become parallel branches of the graph.
 bool launch_all(reproblas_hip_batch* b, hipStream_t s) {
+  const bool fork = !b->segs.empty() && !b->gemms.empty();
+  hipStream_t gs = s;
+  if (fork) {
+    if (!b->side && !RB_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking))) return false;
+    if (!b->ev_fork && !RB_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming))) return false;
+    if (!b->ev_join && !RB_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming))) return false;
+    if (!RB_HIP(hipEventRecord(b->ev_fork, s))) return false;
+    if (!RB_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0))) return false;
+    gs = b->side;
+  }
+  bool ok = true;
   for (const SegGroup& g : b->segs)
-    if (!segments(b->fold, g.term, g.x, g.y, g.d_chunks, g.nchunks, g.d_partials, g.d_first, g.nseg, g.results,
-                  g.grid, s))
-      return false;
+    if (ok && !segments(b->fold, g.term, g.x, g.y, g.d_chunks, g.nchunks, g.d_partials, g.d_first, g.nseg,
+                        g.results, g.grid, s))
+      ok = false;
   for (const GemmGroup& g : b->gemms)
-    if (!gemm_batch(b->fold, g.tiling, g.d_descs, g.desc0, g.count, g.max_work, g.d_tiles, g.ntiles, g.grid, s))
-      return false;
-  return true;
+    if (ok && !gemm_batch(b->fold, g.tiling, g.d_descs, g.desc0, g.count, g.max_work, g.d_tiles, g.ntiles, g.grid,
+                          gs))
+      ok = false;
+  if (fork) {
+    // join even after a failed launch, so a capture in progress stays whole
+    if (!RB_HIP(hipEventRecord(b->ev_join, gs))) return false;
+    if (!RB_HIP(hipStreamWaitEvent(s, b->ev_join, 0))) return false;
+  }
+  return ok;
 }
 
 void drop_graph(reproblas_hip_batch* b) {
@@ -223,7 +251,7 @@ int reproblas_hip_batch_add_gemms(reproblas_hip_batch* b, char Order, char Trans
   if (!g.d_descs || !g.d_tiles) return last_error();
   const DeviceInfo* d = device_info();
   if (!d) return last_error();
-  g.grid = (int)std::min<int64_t>(g.ntiles, (int64_t)d->cus * 4);
+  g.grid = (int)std::min<int64_t>(g.ntiles, INT_MAX);  // one workgroup per tile
   b->gemms.push_back(g);
   return kOk;
 }
@@ -271,6 +299,9 @@ void reproblas_hip_batch_destroy(reproblas_hip_batch* b) {
   (void)hipDeviceSynchronize();
   drop_graph(b);
   if (b->capture) (void)hipStreamDestroy(b->capture);
+  if (b->side) (void)hipStreamDestroy(b->side);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   for (SegGroup& g : b->segs) {
     (void)hipFree(g.d_chunks);
     (void)hipFree(g.d_first);

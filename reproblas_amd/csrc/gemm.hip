@@ -24,6 +24,8 @@
 //     the caller's binned C), converts and writes.
 // Non-finite products poison an output's top primary; the epilogue then
 // recomputes that output's class from its K terms (NaN, +inf or -inf).
+#include <limits.h>
+
 #include "binned_core.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -304,19 +306,19 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc g) {
   gemm_tile<CAP, FIXED, TM, TN, AI, BK>(g, (int64_t)blockIdx.y * 16 * TM, (int64_t)blockIdx.x * 16 * TN, As, Bs);
 }
 
-// Ragged gemm batch: a persistent grid walks a tile list (gemm, tile row,
-// tile column), heaviest K first, each workgroup taking every gridDim.x-th.
+// Ragged gemm batch: one workgroup per entry of a tile list (gemm, tile row,
+// tile column) sorted heaviest K first, so the longest tiles are dispatched
+// first. (One tile per workgroup keeps the register count of gemm_kernel; a
+// persistent loop over tiles costs the compiler ~80 more VGPRs.)
 template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK>
 __global__ __launch_bounds__(kThreads) void gemm_batch_kernel(const GemmDesc* descs, const GemmTile* tiles,
                                                               int64_t ntiles) {
   __shared__ __attribute__((aligned(16))) double As[KC][16 * TM + kPad];
   __shared__ __attribute__((aligned(16))) double Bs[KC][16 * TN + kPad];
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const GemmTile tl = tiles[t];
-    const GemmDesc g = descs[tl.gemm];
-    gemm_tile<CAP, FIXED, TM, TN, AI, BK>(g, (int64_t)tl.ti * 16 * TM, (int64_t)tl.tj * 16 * TN, As, Bs);
-    __syncthreads();  // the next tile reuses the LDS tiles
-  }
+  if ((int64_t)blockIdx.x >= ntiles) return;
+  const GemmTile tl = tiles[blockIdx.x];
+  const GemmDesc g = descs[tl.gemm];
+  gemm_tile<CAP, FIXED, TM, TN, AI, BK>(g, (int64_t)tl.ti * 16 * TM, (int64_t)tl.tj * 16 * TN, As, Bs);
 }
 
 // Chunk maxima for every gemm of a batch (blockIdx.y = gemm).
@@ -456,6 +458,11 @@ bool gemm_batch(int fold, int tiling, const GemmDesc* d_descs, const GemmDesc& d
     return false;
   }
   if (count <= 0 || ntiles <= 0) return true;
+  (void)grid;  // one workgroup per tile
+  if (ntiles > INT_MAX) {
+    set_error(kBadArgument, "gemm batch of %lld tiles exceeds one launch", (long long)ntiles);
+    return false;
+  }
   int64_t mgrid = (max_work + 255) / 256;
   if (mgrid > 1024) mgrid = 1024;
   if (mgrid < 1) mgrid = 1;
@@ -463,12 +470,12 @@ bool gemm_batch(int fold, int tiling, const GemmDesc* d_descs, const GemmDesc& d
   if (!RB_HIP(hipGetLastError())) return false;
   const bool ai = (desc0.sai == 1), bk = (desc0.sbk == 1);
   switch (fold) {
-    case 2: return launch_batch_fold<2, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
-    case 3: return launch_batch_fold<3, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
-    case 4: return launch_batch_fold<4, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
+    case 2: return launch_batch_fold<2, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, (int)ntiles, s);
+    case 3: return launch_batch_fold<3, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, (int)ntiles, s);
+    case 4: return launch_batch_fold<4, true>(tiling, ai, bk, d_descs, d_tiles, ntiles, (int)ntiles, s);
     default:
-      if (fold <= 8) return launch_batch_fold<8, false>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
-      return launch_batch_fold<kMaxFold, false>(tiling, ai, bk, d_descs, d_tiles, ntiles, grid, s);
+      if (fold <= 8) return launch_batch_fold<8, false>(tiling, ai, bk, d_descs, d_tiles, ntiles, (int)ntiles, s);
+      return launch_batch_fold<kMaxFold, false>(tiling, ai, bk, d_descs, d_tiles, ntiles, (int)ntiles, s);
   }
 }
 
