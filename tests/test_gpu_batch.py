@@ -131,6 +131,52 @@ def test_gemm_batch_matches_single_calls(gpu, tiling, beta):
     b.close()
 
 
+@pytest.mark.parametrize("tiling", [1, 2])
+def test_mixed_plan_segments_and_gemms(gpu, tiling):
+    """A plan with segment groups and a gemm group runs the gemms on a forked
+    branch (a side stream, a parallel graph branch): every result still equals
+    its single call, on direct launches and on graph replays."""
+    import torch
+
+    from reproblas_amd.batch import Batch
+
+    off = ragged_offsets(200, 21)
+    n = int(off[-1])
+    x = dg.wide(n, 5)
+    y = dg.wide(n, 6)
+    tx, ty = torch.from_numpy(x).to(gpu), torch.from_numpy(y).to(gpu)
+    rdot = torch.full((len(off) - 1,), -1.0, dtype=torch.float64, device=gpu)
+    rsum = torch.full((len(off) - 1,), -1.0, dtype=torch.float64, device=gpu)
+    sizes, triples, host = _gemm_problem(10, 23, gpu, True)
+    O = oracle()
+    exp_dot = np.array([O.rddot(3, x[off[s]:off[s + 1]], y[off[s]:off[s + 1]]) for s in range(len(off) - 1)])
+    exp_sum = np.array([O.rdsum(3, np.ascontiguousarray(x[off[s]:off[s + 1]])) for s in range(len(off) - 1)])
+    exp_c = []
+    for (M, N, K), (A, B, C) in zip(sizes, host):
+        Ce = C.copy()
+        O.fn["rdgemm"](3, b"C", b"N", b"N", int(M), int(N), int(K), 0.75, vp(A), int(M), vp(B), int(K), 2.0,
+                       vp(Ce), int(M))
+        exp_c.append(Ce)
+    b = Batch(3)
+    b.add_segments("dot", off, tx, ty, rdot)
+    b.add_gemms("N", "N", triples, alpha=0.75, beta=2.0, tiling=tiling)
+    b.add_segments("sum", off, tx, None, rsum, chunk=1024)
+    for rep, graph in enumerate((False, True, True, True, False)):
+        rdot.fill_(-1.0)
+        rsum.fill_(-1.0)
+        for (_, _, tC), (_, _, C) in zip(triples, host):
+            tC.copy_(torch.from_numpy(C))
+        _ = torch.sum(tx * 2.0)  # eager work between replays
+        torch.cuda.synchronize()
+        b.launch(graph=graph)
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(rdot), exp_dot.view(np.uint64)), rep
+        assert np.array_equal(bits(rsum), exp_sum.view(np.uint64)), rep
+        for (_, _, tC), Ce in zip(triples, exp_c):
+            assert np.array_equal(bits(tC), Ce.view(np.uint64)), rep
+    b.close()
+
+
 def test_autotuner_picks_a_valid_config(gpu, tmp_path, monkeypatch):
     import torch
 
