@@ -91,6 +91,55 @@ def cpu_baseline(x_sample, y_sample):
                       f"{t_total:.1f} s"}, res
 
 
+def cpu_baseline_parallel(x_sample, y_sample, ref_value):
+    """The same sample sharded over the host's cores: each thread runs the
+    reference's binnedBLAS_dbddot on its shard (ctypes releases the GIL), the
+    binned shards merge in order with binned_dbdbadd and binned_ddbconv
+    converts; the bits equal the single-thread rddot."""
+    import concurrent.futures
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from cpu_libs import oracle, reference
+
+    lib = reference("avx")
+    kind = "reference"
+    if lib is None:
+        lib, kind = oracle(), "port"
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(16, ncpu))
+    n = len(x_sample)
+    bounds = [n * i // threads for i in range(threads + 1)]
+    parts = [np.zeros(6) for _ in range(threads)]
+    dbddot = lib.fn["dbddot"]
+    vpa = lambda a: ctypes.c_void_p(a.ctypes.data)
+
+    def shard(i):
+        parts[i][:] = 0.0
+        lo, hi = bounds[i], bounds[i + 1]
+        dbddot(3, hi - lo, ctypes.c_void_p(x_sample.ctypes.data + 8 * lo), 1,
+               ctypes.c_void_p(y_sample.ctypes.data + 8 * lo), 1, vpa(parts[i]))
+
+    reps, t_total, res = 0, 0.0, None
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        while t_total < 5.0 and reps < 20000:
+            t = time.perf_counter()
+            list(ex.map(shard, range(threads)))
+            acc = parts[0].copy()
+            for i in range(1, threads):
+                lib.fn["dbdbadd"](3, vpa(parts[i]), vpa(acc))
+            res = lib.fn["ddbconv"](3, vpa(acc))
+            t_total += time.perf_counter() - t
+            reps += 1
+    gbs = 16.0 * n * reps / t_total / 1e9
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": kind,
+            "sample": f"the same {n}-term sample in {threads} shards (dbddot per thread, dbdbadd merge), "
+                      f"{reps} repetitions, {t_total:.1f} s",
+            "bits_equal_single_thread": bits(res) == bits(ref_value)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,7 +190,7 @@ def main():
     log(f"[bench] ddot {t * 1e3:.3f} ms/call, {value:.1f} GB/s")
 
     # bit-exactness spot check on a sample against the CPU reference
-    cpu = None
+    cpu = cpu_par = None
     ref_sample = gpu_sample = None
     if not args.no_cpu:
         ns = 1 << 24
@@ -152,6 +201,7 @@ def main():
         gpu_sample = float(out[1].item())
         log("[bench] CPU baseline")
         cpu, ref_sample = cpu_baseline(xs, ys)
+        cpu_par = cpu_baseline_parallel(xs, ys, ref_sample)
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic_blas1_ddot_2p28.json")
     if os.path.exists(tp):
@@ -175,6 +225,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(value, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(value / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_par,
         "parity_sample": None if ref_sample is None else (bits(ref_sample) == bits(gpu_sample)),
     }
     del x, y

@@ -12,7 +12,7 @@ for p in (HERE, ROOT):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
-    config.addinivalue_line("markers", "slow: full-size (BASELINE-scale) GPU case")
+    config.addinivalue_line("markers", "slow: full-size (benchmark-scale) GPU case")
 
 
 @pytest.fixture(scope="session")

@@ -89,7 +89,7 @@ def test_every_gemm_call_matches_cpu_reference(gpu):
 
 @pytest.mark.slow
 def test_full_mlp_identical_across_tilings(gpu):
-    """BASELINE scale: 1024-4096-4096-1024, batch 2048."""
+    """Benchmark scale: 1024-4096-4096-1024, batch 2048."""
     import torch
 
     import reproblas_amd as rb

@@ -248,7 +248,7 @@ def test_damax(gpu):
 @pytest.mark.slow
 @pytest.mark.parametrize("fold", [2, 3, 4])
 def test_full_size_dot_nrm2(gpu, fold):
-    """BASELINE scale: N = 2^28 wide-range dot and nrm2, bit-identical to the oracle."""
+    """Benchmark scale: N = 2^28 wide-range dot and nrm2, bit-identical to the oracle."""
     import torch
 
     import reproblas_amd as rb
@@ -262,6 +262,40 @@ def test_full_size_dot_nrm2(gpu, fold):
     assert same(rb.rdnrm2(tx, fold), O.rdnrm2(fold, x))
     if fold == 3:
         assert same(rb.rdsum(tx, fold), O.rdsum(fold, x))
+
+
+@pytest.mark.slow
+def test_full_size_ill_conditioned_sum(gpu):
+    """N = 2^28 ill-conditioned sum: N/4 values x, their N/4 negations and N/2
+    terms of 1e-12*U(-1,1) noise, in three random orders. The three sums are
+    bit-identical to each other and to the oracle (order independence at full
+    size)."""
+    import torch
+
+    import reproblas_amd as rb
+
+    n = 1 << 28
+    g = torch.Generator(device=gpu)
+    g.manual_seed(77)
+    q = n // 4
+    v = (torch.rand(q, generator=g, device=gpu, dtype=torch.float64) + 1.0) * torch.ldexp(
+        torch.ones(q, device=gpu, dtype=torch.float64),
+        torch.randint(-40, 41, (q,), generator=g, device=gpu, dtype=torch.int32))
+    noise = (torch.rand(n - 2 * q, generator=g, device=gpu, dtype=torch.float64) * 2 - 1) * 1e-12
+    base = torch.cat([v, -v, noise])
+    del v, noise
+    sums = []
+    for seed in (1, 2, 3):
+        g.manual_seed(seed)
+        perm = torch.randperm(n, generator=g, device=gpu)
+        xs = base[perm]
+        del perm
+        sums.append(rb.rdsum(xs))
+        if seed == 1:
+            host = xs.cpu().numpy()
+        del xs
+    assert same(sums[0], sums[1]) and same(sums[0], sums[2])
+    assert same(sums[0], oracle().rdsum(3, host))
 
 
 @pytest.mark.parametrize("fold", [9, 13, 26, 52])

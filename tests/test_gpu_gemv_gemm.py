@@ -144,7 +144,7 @@ def test_gemm_binned_accumulate(gpu):
 
 @pytest.mark.slow
 def test_gemv_full_size(gpu):
-    """BASELINE scale 16384^2 column-major N and T, fold 3, beta in {0, 0.5}."""
+    """Benchmark scale: 16384^2 column-major N and T, fold 3, beta in {0, 0.5}."""
     import torch
 
     import reproblas_amd as rb
