@@ -5,9 +5,10 @@ N = 2^28 FP64 with wide dynamic range x = s*m*2^e (s = +-1, m ~ U[1,2),
 e ~ U{-40..40}), both operands resident in HBM; GB/s of algorithmic traffic
 (16 bytes per term). A "step" is one ddot over the whole vector pair.
 
-Also reported under "extra": dsum 2^20 (uniform), dsum / dnrm2 2^28, dgemv
-16384^2 (N and T), dgemm 4096^3 (GFLOP/s and fraction of the FP64 VALU issue
-rate), each as one timed configuration.
+Also reported under "extra": dsum 2^20 (uniform), dsum / dnrm2 2^28, ddot and
+dnrm2 2^28 at folds 2, 3, 4 and dsum with incX = 2, dgemv 16384^2 (N and T),
+dgemm 4096^3 (GFLOP/s and fraction of the FP64 VALU issue rate), the ragged
+batch and the MLP step, each as one timed configuration.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -147,7 +148,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configurations)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--only", choices=["dsum_2p20", "dsum_2p28", "dnrm2_2p28", "dgemv", "dgemm", "ragged_batch", "mlp"],
+    ap.add_argument("--only", choices=["dsum_2p20", "dsum_2p28", "dnrm2_2p28", "folds", "dgemv", "dgemm", "ragged_batch",
+                                       "mlp"],
                     help="run one extra configuration alone and print its record (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy and a plain torch sum of 2^28 doubles (scales the HBM counters)")
@@ -283,6 +285,27 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
         del x
         torch.cuda.empty_cache()
 
+    if want("folds"):
+        # ddot and dnrm2 at folds 2, 3, 4 (N = 2^28 wide-range), and dsum with incX = 2
+        n = 1 << 28
+        x = wide_gpu(n, 1, dev)
+        y = wide_gpu(n, 2, dev)
+        rec = {"ddot": {}, "dnrm2": {}}
+        for fold in (2, 3, 4):
+            t = time_launches(lambda: L.reproblas_hip_rddot_device(fold, n, vp(x), 1, vp(y), 1, vp(out), sp),
+                              steps, 2, stream)
+            rec["ddot"][str(fold)] = {"ms": round(t * 1e3, 3), "GB/s": round(16 * n / t / 1e9, 1)}
+            t = time_launches(lambda: L.reproblas_hip_rdnrm2_device(fold, n, vp(x), 1, vp(out), sp), steps, 2, stream)
+            rec["dnrm2"][str(fold)] = {"ms": round(t * 1e3, 3), "GB/s": round(8 * n / t / 1e9, 1)}
+        h = n // 2
+        t = time_launches(lambda: L.reproblas_hip_rdsum_device(3, h, vp(x), 2, vp(out), sp), steps, 2, stream)
+        rec["dsum_inc2"] = {"terms": h, "ms": round(t * 1e3, 3), "GB/s_of_terms": round(8 * h / t / 1e9, 1),
+                            "GB/s_of_span": round(16 * h / t / 1e9, 1)}
+        ex["folds"] = ex["folds_2p28"] = rec
+        log(f"[bench] folds: {rec}")
+        del x, y
+        torch.cuda.empty_cache()
+
     if want("dgemv"):
         # dgemv 16384^2 column-major, fold 3, alpha 1, beta 0.5
         m = 16384
@@ -323,6 +346,7 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
     if want("mlp"):
         ex["mlp"] = ex["mlp_fwd_bwd"] = mlp(dev, stream, args)
     if only is None:
+        ex.pop("folds", None)
         ex.pop("dgemv", None)
         ex.pop("dgemm", None)
         ex.pop("mlp", None)
