@@ -33,6 +33,37 @@ namespace {
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
 
+// Partner value for stage `st` (0..5) of a wave reduction tree. Stage st pairs
+// every lane with one lane of the other half of its aligned group of
+// 2^(st+1) lanes; when every lane combines its value with its partner's by an
+// exact, commutative operation, after stage st all lanes of such a group hold
+// the same value, and after stage 5 every lane holds the whole wave's. Stages
+// 0-3 are DPP lane moves inside rows of 16 (quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror: these pair lane i with the other
+// half of its group because the groups below already agree), stage 4 a
+// ds_swizzle xor 16 inside 32 lanes, stage 5 reads lanes 0 and 32.
+__device__ __forceinline__ int lane_partner32(int v, int st) {
+  switch (st) {
+    case 0: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);
+    case 1: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);
+    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);
+    case 3: return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true);
+    case 4: return __builtin_amdgcn_ds_swizzle(v, 0x401F);
+    default: {
+      const int lo_half = __builtin_amdgcn_readlane(v, 0);
+      const int hi_half = __builtin_amdgcn_readlane(v, 32);
+      return (__lane_id() < 32) ? hi_half : lo_half;
+    }
+  }
+}
+
+__device__ __forceinline__ double lane_partner(double v, int st) {
+  const uint64_t u = bits(v);
+  const uint32_t lo = (uint32_t)lane_partner32((int)(uint32_t)u, st);
+  const uint32_t hi = (uint32_t)lane_partner32((int)(uint32_t)(u >> 32), st);
+  return from_bits(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ double uniform_double(double v) {
@@ -44,7 +75,7 @@ __device__ __forceinline__ double uniform_double(double v) {
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, shfl_xor_d(v, m));
+  for (int st = 0; st < 6; ++st) v = fmax(v, lane_partner(v, st));
   return v;
 }
 
@@ -178,15 +209,14 @@ RB_UNROLL_CAP
 template <int CAP, int T>
 __device__ __forceinline__ void wave_merge_general(int fold, State<CAP>& s) {
 #pragma unroll
-  for (int m = 1; m <= 32; m <<= 1) {
+  for (int st = 0; st < 6; ++st) {
     State<CAP> o;
 RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
-      o.p[j] = shfl_xor_d(s.p[j], m);
-      o.c[j] = shfl_xor_d(s.c[j], m);
+      o.p[j] = lane_partner(s.p[j], st);
+      o.c[j] = lane_partner(s.c[j], st);
     }
-    o.scl = shfl_xor_d(s.scl, m);
-    // lower lane of each pair keeps (y = lower, x = upper) so both lanes agree
+    o.scl = lane_partner(s.scl, st);
     merge<CAP, T>(fold, s, o);
   }
 }
@@ -310,12 +340,12 @@ __device__ __forceinline__ void wave_finish(const Span& a, int fold, int64_t t0,
   renorm(fold, s.p, s.c);
   s.scl = w.scl;
 #pragma unroll
-  for (int m = 1; m <= 32; m <<= 1) {
+  for (int st = 0; st < 6; ++st) {
     double op[CAP], oc[CAP];
 RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
-      op[j] = shfl_xor_d(s.p[j], m);
-      oc[j] = shfl_xor_d(s.c[j], m);
+      op[j] = lane_partner(s.p[j], st);
+      oc[j] = lane_partner(s.c[j], st);
     }
     if (!nan_or_inf(s.p[0]) && !nan_or_inf(op[0])) {
       add_same_index(fold, w.idx, s.p, s.c, op, oc);
