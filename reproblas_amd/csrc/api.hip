@@ -223,6 +223,12 @@ static int device_blas1(int fold, Term term, int64_t N, const double* X, int64_t
   clear_error();
   if (!check_fold(fold)) return last_error();
   if (N < 0) N = 0;
+  // the kernels dereference these directly: refuse host memory up front
+  if (N > 0 && (!require_device_accessible(X, "X") || (term == kDot && !require_device_accessible(Y, "Y"))))
+    return last_error();
+  if (!require_device_accessible(sin, "binned state", true) ||
+      !require_device_accessible(scale_in_dev, "scale", true) || !require_device_accessible(result, "result", true))
+    return last_error();
   if (!blas1(fold, term, N, X, incX, Y, incY, sin, 0.0, scale_in_dev, sout, scale_out, result,
              (hipStream_t)stream))
     return last_error();
@@ -387,6 +393,9 @@ int reproblas_hip_rdgemv_device(int fold, char Order, char TransA, int64_t M, in
   clear_error();
   if (M <= 0 || N <= 0) return kOk;
   if (!check_fold(fold)) return last_error();
+  if (!require_device_accessible(Y, "Y") ||
+      (alpha != 0.0 && (!require_device_accessible(A, "A") || !require_device_accessible(X, "X"))))
+    return last_error();
   if (!rdgemv_device(fold, Order, TransA, M, N, alpha, A, lda, X, incX, beta, Y, incY, (hipStream_t)stream))
     return last_error();
   return kOk;
@@ -398,6 +407,9 @@ int reproblas_hip_rdgemm_device(int fold, char Order, char TransA, char TransB, 
   clear_error();
   if (M <= 0 || N <= 0) return kOk;
   if (!check_fold(fold)) return last_error();
+  if (!require_device_accessible(C, "C") ||
+      (alpha != 0.0 && K > 0 && (!require_device_accessible(A, "A") || !require_device_accessible(B, "B"))))
+    return last_error();
   if (!rdgemm_device(fold, Order, TransA, TransB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                      (hipStream_t)stream))
     return last_error();

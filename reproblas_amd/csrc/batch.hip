@@ -157,6 +157,9 @@ int reproblas_hip_batch_add_segments(reproblas_hip_batch* b, int op, int64_t nse
     set_error(kBadArgument, "reproblas_hip_batch_add_segments: bad argument");
     return last_error();
   }
+  if (nseg > 0 && (!require_device_accessible(X, "X") || (op == kDot && !require_device_accessible(Y, "Y")) ||
+                   !require_device_accessible(results, "results")))
+    return last_error();
   if (chunk <= 0) chunk = 16384;
   chunk = (chunk + 127) / 128 * 128;
   drop_graph(b);
@@ -221,6 +224,9 @@ int reproblas_hip_batch_add_gemms(reproblas_hip_batch* b, char Order, char Trans
                 "are required in a batch; call reproBLAS_rdgemm for it)", i);
       return last_error();
     }
+    if (!require_device_accessible(A[i], "A") || !require_device_accessible(B[i], "B") ||
+        !require_device_accessible(C[i], "C"))
+      return last_error();
     descs[i] = make_gemm_desc(b->fold, Order, TransA, TransB, M[i], N[i], K[i], alpha[i], A[i], lda[i], B[i], ldb[i],
                               beta[i], C[i], ldc[i], nullptr, 0);
     maxima += ((K[i] + 31) / 32) * (M[i] + N[i]);
@@ -235,7 +241,7 @@ int reproblas_hip_batch_add_gemms(reproblas_hip_batch* b, char Order, char Trans
     descs[i].colmax = g.d_maxima + off + ch * M[i];
     off += ch * (M[i] + N[i]);
   }
-  // tiles, heaviest (largest K) first so the persistent grid ends evenly
+  // tiles, heaviest (largest K) first so the longest tiles are dispatched first
   int bm = 64, bn = 32;
   gemm_tile_shape(tiling, b->fold, &bm, &bn);
   std::vector<GemmTile> tiles;

@@ -132,4 +132,30 @@ bool is_device_pointer(const void* p) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
+bool is_device_accessible(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  memset(&attr, 0, sizeof(attr));
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) return true;
+  if (attr.type != hipMemoryTypeHost || attr.devicePointer == nullptr) return false;
+  // pinned host memory: the runtime must also hand out a device address for it
+  void* dp = nullptr;
+  if (hipHostGetDevicePointer(&dp, const_cast<void*>(p), 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return dp != nullptr;
+}
+
+bool require_device_accessible(const void* p, const char* what, bool may_be_null) {
+  if (!p && may_be_null) return true;
+  if (is_device_accessible(p)) return true;
+  set_error(kBadArgument, "%s is not device-accessible memory (device, managed or pinned host)", what);
+  return false;
+}
+
 }  // namespace rb

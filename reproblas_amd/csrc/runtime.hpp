@@ -54,6 +54,12 @@ unsigned* tickets(hipStream_t s, int count);
 
 // Pointer kind of `p`: true when a kernel may dereference it directly.
 bool is_device_pointer(const void* p);
+// True when a kernel may dereference p: device or managed memory, or pinned
+// host memory mapped for the GPU. Pageable host memory is not.
+bool is_device_accessible(const void* p);
+// Sets kBadArgument naming `what` and returns false unless p is device
+// accessible (null passes when `may_be_null`).
+bool require_device_accessible(const void* p, const char* what, bool may_be_null = false);
 
 enum WorkspaceSlot : int {
   kSlotPartials = 0,

@@ -236,6 +236,80 @@ def test_device_result_api(gpu):
         assert same(g_, e_)
 
 
+def test_device_api_refuses_host_memory(gpu):
+    """The *_device calls dereference their pointers in kernels; pageable host
+    memory is refused on the host with an error (nothing is launched)."""
+    import ctypes
+
+    import torch
+
+    import reproblas_amd as rb
+
+    L = rb.lib()
+    x = dg.wide(1000, 43)
+    tx = torch.from_numpy(x).to(gpu)
+    out = torch.zeros(2, dtype=torch.float64, device=gpu)
+    host_out = np.zeros(2)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    dp = lambda t: ctypes.c_void_p(t.data_ptr())
+    hp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    assert L.reproblas_hip_rdsum_device(3, 1000, hp(x), 1, dp(out), s) != 0
+    assert b"device-accessible" in L.reproblas_hip_last_error_string()
+    assert L.reproblas_hip_rddot_device(3, 1000, dp(tx), 1, hp(x), 1, dp(out), s) != 0
+    assert L.reproblas_hip_rdsum_device(3, 1000, dp(tx), 1, hp(host_out), s) != 0
+    A = dg.normal(64 * 64, 44)
+    tA = torch.from_numpy(A).to(gpu)
+    C = torch.zeros(64 * 64, dtype=torch.float64, device=gpu)
+    assert L.reproblas_hip_rdgemm_device(3, b"C", b"N", b"N", 64, 64, 64, 1.0, hp(A), 64, dp(tA), 64, 0.0, dp(C),
+                                         64, s) != 0
+    assert L.reproblas_hip_rdgemv_device(3, b"C", b"N", 64, 64, 1.0, dp(tA), 64, hp(x), 1, 0.0, dp(C), 1, s) != 0
+    rb.lib().reproblas_hip_clear_error()
+    # the same calls on device memory still work afterwards
+    assert L.reproblas_hip_rdsum_device(3, 1000, dp(tx), 1, dp(out), s) == 0
+    assert same(out[0].item(), oracle().rdsum(3, x))
+
+
+def test_checkpoint_resume(gpu, tmp_path):
+    """A binned accumulator is a checkpoint: accumulate part of a stream on the
+    device, save the state to disk, reload it (in a fresh buffer) and finish;
+    the bits equal one uninterrupted call, on the device and host APIs."""
+    import ctypes
+
+    import torch
+
+    import reproblas_amd as rb
+
+    L = rb.lib()
+    n = 300001
+    x, y = dg.wide(n, 45), dg.wide(n, 46)
+    tx, ty = torch.from_numpy(x).to(gpu), torch.from_numpy(y).to(gpu)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    dp = lambda t, off=0: ctypes.c_void_p(t.data_ptr() + 8 * off)
+    for fold in (2, 3, 5):
+        Y = torch.zeros(2 * fold, dtype=torch.float64, device=gpu)
+        assert L.reproblas_hip_dbddot_device(fold, 123457, dp(tx), 1, dp(ty), 1, dp(Y), s) == 0
+        path = tmp_path / f"acc{fold}.npy"
+        np.save(path, Y.cpu().numpy())
+        del Y
+        Y2 = torch.from_numpy(np.load(path)).to(gpu)
+        assert L.reproblas_hip_dbddot_device(fold, n - 123457, dp(tx, 123457), 1, dp(ty, 123457), 1, dp(Y2), s) == 0
+        one = torch.zeros(2 * fold, dtype=torch.float64, device=gpu)
+        assert L.reproblas_hip_dbddot_device(fold, n, dp(tx), 1, dp(ty), 1, dp(one), s) == 0
+        assert np.array_equal(Y2.cpu().numpy().view(np.uint64), one.cpu().numpy().view(np.uint64)), fold
+        Ye = np.zeros(2 * fold)
+        oracle().dbddot(fold, x, y, Ye)
+        assert np.array_equal(one.cpu().numpy().view(np.uint64), Ye.view(np.uint64)), fold
+        # host API: the same checkpoint taken through binnedBLAS_dbdsum
+        H = np.zeros(2 * fold)
+        rb.dbdsum(H, x[:777], fold)
+        np.save(path, H)
+        H2 = np.load(path)
+        rb.dbdsum(H2, x[777:], fold)
+        He = np.zeros(2 * fold)
+        oracle().dbdsum(fold, x, He)
+        assert np.array_equal(H2.view(np.uint64), He.view(np.uint64)), fold
+
+
 def test_damax(gpu):
     import reproblas_amd as rb
 
