@@ -24,6 +24,11 @@ extern "C" {
 #define ORC_MAXINDEX 51
 #define ORC_MAXFOLD 52
 #define ORC_ENDURANCE 2048
+/* single precision (reference include/binned.h:95,115,131,153) */
+#define ORC_SBWIDTH 13
+#define ORC_SMAXINDEX 20
+#define ORC_SMAXFOLD 21
+#define ORC_SENDURANCE 512
 
 /* primitives (reference src/binned/) */
 const double *orc_dmbins(int X);
@@ -73,6 +78,45 @@ void orc_dbdgemv(int fold, char Order, char TransA, int M, int N, double alpha,
 void orc_dbdgemm(int fold, char Order, char TransA, char TransB, int M, int N, int K,
                  double alpha, const double *A, int lda, const double *B, int ldb,
                  double *C, int ldc);
+
+/* ------------------------------------------------ single precision (oracle_s.c) */
+const float *orc_smbins(int X);
+int orc_sindex(float X);
+int orc_smindex(const float *priX);
+int orc_smindex0(const float *priX);
+void orc_smsupdate(int fold, float X, float *priY, int incpriY, float *carY, int inccarY);
+void orc_smsdeposit(int fold, float X, float *priY, int incpriY);
+void orc_smrenorm(int fold, float *priX, int incpriX, float *carX, int inccarX);
+void orc_smsmadd(int fold, const float *priX, int incpriX, const float *carX, int inccarX,
+                 float *priY, int incpriY, float *carY, int inccarY);
+void orc_sbsbadd(int fold, const float *X, float *Y);
+void orc_smsadd(int fold, float X, float *priY, int incpriY, float *carY, int inccarY);
+void orc_sbsadd(int fold, float X, float *Y);
+void orc_sbsconv(int fold, float X, float *Y);
+float orc_ssmconv(int fold, const float *priX, int incpriX, const float *carX, int inccarX);
+float orc_ssbconv(int fold, const float *X);
+float orc_sscale(float X);
+void orc_smsrescale(int fold, float X, float scaleY, float *priY, int incpriY, float *carY, int inccarY);
+float orc_sbsbaddsq(int fold, float scaleX, const float *X, float scaleY, float *Y);
+float orc_sbbound(int fold, int N, float X, float S);
+float orc_samax(int N, const float *X, int incX);
+float orc_samaxm(int N, const float *X, int incX, const float *Y, int incY);
+void orc_sbssum(int fold, int N, const float *X, int incX, float *Y);
+void orc_sbsasum(int fold, int N, const float *X, int incX, float *Y);
+void orc_sbsdot(int fold, int N, const float *X, int incX, const float *Y, int incY, float *Z);
+float orc_sbsssq(int fold, int N, const float *X, int incX, float scaleY, float *Y);
+float orc_rssum(int fold, int N, const float *X, int incX);
+float orc_rsasum(int fold, int N, const float *X, int incX);
+float orc_rsdot(int fold, int N, const float *X, int incX, const float *Y, int incY);
+float orc_rsnrm2(int fold, int N, const float *X, int incX);
+void orc_sbsgemv(int fold, char Order, char TransA, int M, int N, float alpha, const float *A, int lda,
+                 const float *X, int incX, float *Y, int incY);
+void orc_rsgemv(int fold, char Order, char TransA, int M, int N, float alpha, const float *A, int lda,
+                const float *X, int incX, float beta, float *Y, int incY);
+void orc_sbsgemm(int fold, char Order, char TransA, char TransB, int M, int N, int K, float alpha, const float *A,
+                 int lda, const float *B, int ldb, float *C, int ldc);
+void orc_rsgemm(int fold, char Order, char TransA, char TransB, int M, int N, int K, float alpha, const float *A,
+                int lda, const float *B, int ldb, float beta, float *C, int ldc);
 
 #ifdef __cplusplus
 }

@@ -1,10 +1,11 @@
 """ctypes handles on the CPU checker libraries (test infrastructure only).
 
-* ``oracle()``  -- oracle/liboracle.so, the scalar restatement (orc_* symbols).
+* ``oracle()``  -- oracle/liboracle.so, the scalar restatement (orc_* symbols);
+  ``oracle_s()`` the same library's single-precision calls.
 * ``reference(kind)`` -- oracle/_ref/libreproblas_ref{,_sisd}.so, the
   reference's own C sources compiled by oracle/Makefile; ``None`` when they
   were not built (the reference tree is only present where the fixtures are
-  made).
+  made); ``reference_s(kind)`` its single-precision calls.
 
 Both are wrapped in :class:`CpuBlas`, which exposes the reproBLAS-style calls
 on numpy arrays so a test can run the same call on either library.
@@ -160,6 +161,87 @@ class CpuBlas:
                            _ptr(B), ldb, _ptr(C), ldc)
 
 
+_c_float = ctypes.c_float
+
+
+def _fptr(a):
+    if not a.flags["C_CONTIGUOUS"] or a.dtype != np.float32:
+        raise ValueError("CPU checker calls need contiguous float32 arrays")
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class CpuBlasS:
+    """The single-precision (float_binned) calls on numpy float32 arrays."""
+
+    def __init__(self, path, style):
+        self.path = path
+        self.style = style
+        self.lib = ctypes.CDLL(path)
+        f, i, c, v = _c_float, _c_int, _c_char, _vp
+        sig = {
+            "rssum": (f, [i, i, v, i]), "rsasum": (f, [i, i, v, i]), "rsnrm2": (f, [i, i, v, i]),
+            "rsdot": (f, [i, i, v, i, v, i]),
+            "rsgemv": (None, [i, c, c, i, i, f, v, i, v, i, f, v, i]),
+            "rsgemm": (None, [i, c, c, c, i, i, i, f, v, i, v, i, f, v, i]),
+            "sbssum": (None, [i, i, v, i, v]), "sbsasum": (None, [i, i, v, i, v]),
+            "sbsdot": (None, [i, i, v, i, v, i, v]), "sbsssq": (f, [i, i, v, i, f, v]),
+            "sbsgemv": (None, [i, c, c, i, i, f, v, i, v, i, v, i]),
+            "sbsgemm": (None, [i, c, c, c, i, i, i, f, v, i, v, i, v, i]),
+            "sbsbadd": (None, [i, v, v]), "sbsbaddsq": (f, [i, f, v, f, v]), "sbsconv": (None, [i, f, v]),
+            "ssbconv": (f, [i, v]), "sbsadd": (None, [i, f, v]), "sindex": (i, [f]), "smindex": (i, [v]),
+            "sscale": (f, [f]), "sbbound": (f, [i, i, f, f]), "samax": (f, [i, v, i]),
+            "samaxm": (f, [i, v, i, v, i]),
+        }
+        self.fn = {}
+        for k, (res, args) in sig.items():
+            fn = getattr(self.lib, self._name(k))
+            fn.restype = res
+            fn.argtypes = args
+            self.fn[k] = fn
+
+    def _name(self, k):
+        if self.style == "orc":
+            return "orc_" + k
+        if k.startswith("rs"):
+            return "reproBLAS_" + k
+        if k in ("sbssum", "sbsasum", "sbsdot", "sbsssq", "sbsgemv", "sbsgemm", "samax", "samaxm"):
+            return "binnedBLAS_" + k
+        return "binned_" + k
+
+    def rssum(self, fold, x, inc=1):
+        return self.fn["rssum"](fold, len(x) // abs(inc), _fptr(x), inc)
+
+    def rsasum(self, fold, x, inc=1):
+        return self.fn["rsasum"](fold, len(x) // abs(inc), _fptr(x), inc)
+
+    def rsnrm2(self, fold, x, inc=1):
+        return self.fn["rsnrm2"](fold, len(x) // abs(inc), _fptr(x), inc)
+
+    def rsdot(self, fold, x, y):
+        return self.fn["rsdot"](fold, min(len(x), len(y)), _fptr(x), 1, _fptr(y), 1)
+
+    def sbssum(self, fold, x, Y):
+        self.fn["sbssum"](fold, len(x), _fptr(x), 1, _fptr(Y))
+
+    def sbsasum(self, fold, x, Y):
+        self.fn["sbsasum"](fold, len(x), _fptr(x), 1, _fptr(Y))
+
+    def sbsdot(self, fold, x, y, Z):
+        self.fn["sbsdot"](fold, len(x), _fptr(x), 1, _fptr(y), 1, _fptr(Z))
+
+    def sbsssq(self, fold, x, scale, Y):
+        return self.fn["sbsssq"](fold, len(x), _fptr(x), 1, scale, _fptr(Y))
+
+    def sbsbadd(self, fold, X, Y):
+        self.fn["sbsbadd"](fold, _fptr(X), _fptr(Y))
+
+    def sbsadd(self, fold, x, Y):
+        self.fn["sbsadd"](fold, x, _fptr(Y))
+
+    def ssbconv(self, fold, Y):
+        return self.fn["ssbconv"](fold, _fptr(Y))
+
+
 _cache = {}
 
 
@@ -176,4 +258,20 @@ def reference(kind="avx"):
     if key not in _cache:
         p = REF_SO[kind]
         _cache[key] = CpuBlas(p, "ref") if os.path.exists(p) else None
+    return _cache[key]
+
+
+def oracle_s():
+    if "orc_s" not in _cache:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+        _cache["orc_s"] = CpuBlasS(ORACLE_SO, "orc")
+    return _cache["orc_s"]
+
+
+def reference_s(kind="avx"):
+    key = "ref_s_" + kind
+    if key not in _cache:
+        p = REF_SO[kind]
+        _cache[key] = CpuBlasS(p, "ref") if os.path.exists(p) else None
     return _cache[key]

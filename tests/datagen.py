@@ -77,3 +77,23 @@ def special(kind, n, seed):
     else:
         raise ValueError(kind)
     return x
+
+
+# ---------------------------------------------------------- single precision --
+def wide_s(n, seed, emin=-20, emax=20):
+    """float32 x = s * m * 2^e, s = +-1, m ~ U[1,2) (rounded to float), e ~ U{emin..emax}."""
+    return wide(n, seed, emin, emax).astype(np.float32)
+
+
+def special_s(kind, n, seed):
+    """float32 inputs with non-finite or extreme members."""
+    if kind == "big":  # index-0 (compressed top bin) path of float_binned
+        x = (uniform(n, seed) * np.ldexp(1.0, 125)).astype(np.float32)
+        x[rng(seed + 1000).integers(0, n)] = np.float32(3.4028234663852886e38)
+        return x
+    if kind == "tiny":  # float subnormals and near-underflow
+        return (uniform(n, seed) * np.ldexp(1.0, -135)).astype(np.float32)
+    if kind == "mixed_range":
+        g = rng(seed + 1000)
+        return (uniform(n, seed) * np.exp2(g.integers(-140, 120, n).astype(float))).astype(np.float32)
+    return special(kind, n, seed).astype(np.float32)
