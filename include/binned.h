@@ -1,10 +1,11 @@
 /*
- * binned.h -- the binned double-precision type and its primitives, as
- * exported by libreproblas_hip.so.
+ * binned.h -- the binned double- and single-precision types and their
+ * primitives, as exported by libreproblas_hip.so.
  *
- * Drop-in for the double-precision part of the reference's include/binned.h
- * (ReproBLAS 2.1.0): same type, constants, names, signatures and layout.
- * A double_binned of fold F is double[2F]: F primaries followed by F carries.
+ * Drop-in for the real (double and float) part of the reference's
+ * include/binned.h (ReproBLAS 2.1.0): same types, constants, names,
+ * signatures and layout. A double_binned of fold F is double[2F], a
+ * float_binned float[2F]: F primaries followed by F carries.
  *
  * These primitives act on a few dozen bytes and run on the host, on host
  * memory, exactly where the reference runs them; binned_dbdbaddv also accepts
@@ -94,6 +95,78 @@ double binned_dmdmaddsq(const int fold, const double scaleX, const double *priX,
 double binned_dbdbaddsq(const int fold, const double scaleX, const double_binned *X, const double scaleY,
                         double_binned *Y);                      /* binned.h:373 */
 double binned_ufp(const double X);                              /* binned.h:377 */
+
+/* ------------------------------------------------------------ float_binned
+ * Single precision: float[2F] (F primaries, then F carries), folds 2..21. */
+/* reference include/binned.h:64 */
+typedef float float_binned;
+/* reference include/binned.h:95 -- bin width in bits */
+#define SBWIDTH 13
+/* reference include/binned.h:115 -- maximum index (20) */
+#define binned_SBMAXINDEX (((FLT_MAX_EXP - FLT_MIN_EXP + FLT_MANT_DIG - 1) / SBWIDTH) - 1)
+/* reference include/binned.h:131 -- maximum fold (21) */
+#define binned_SBMAXFOLD (binned_SBMAXINDEX + 1)
+/* reference include/binned.h:153 -- deposits between renormalisations (512) */
+#define binned_SBENDURANCE (1 << (FLT_MANT_DIG - SBWIDTH - 2))
+/* reference include/binned.h:173 */
+#define binned_SBCAPACITY (binned_SBENDURANCE * (1.0 / FLT_EPSILON - 1.0))
+/* reference include/binned.h:195,217 */
+#define binned_SMCOMPRESSION (1.0 / (1 << (FLT_MANT_DIG - SBWIDTH + 1)))
+#define binned_SMEXPANSION (1.0 * (1 << (FLT_MANT_DIG - SBWIDTH + 1)))
+
+size_t binned_sbsbze(const int fold);                           /* binned.h:221 */
+float_binned *binned_sballoc(const int fold);                   /* binned.h:226 */
+int binned_sbnum(const int fold);                               /* binned.h:231 */
+float binned_sbbound(const int fold, const int N, const float X, const float S); /* binned.h:235 */
+const float *binned_smbins(const int X);                        /* binned.h:238 */
+int binned_sindex(const float X);                               /* binned.h:244 */
+int binned_smindex(const float *priX);                          /* binned.h:245 */
+int binned_smindex0(const float *priX);                         /* binned.h:246 */
+int binned_smdenorm(const int fold, const float *priX);         /* binned.h:250 */
+void binned_sbprint(const int fold, const float_binned *X);     /* binned.h:257 */
+void binned_smprint(const int fold, const float *priX, const int incpriX, const float *carX,
+                    const int inccarX);                         /* binned.h:258 */
+void binned_sbsbset(const int fold, const float_binned *X, float_binned *Y); /* binned.h:268 */
+void binned_smsmset(const int fold, const float *priX, const int incpriX, const float *carX, const int inccarX,
+                    float *priY, const int incpriY, float *carY, const int inccarY); /* binned.h:269 */
+void binned_sbsetzero(const int fold, float_binned *X);         /* binned.h:279 */
+void binned_smsetzero(const int fold, float *priX, const int incpriX, float *carX,
+                      const int inccarX);                       /* binned.h:280 */
+void binned_sbsbadd(const int fold, const float_binned *X, float_binned *Y); /* binned.h:288 */
+void binned_smsmadd(const int fold, const float *priX, const int incpriX, const float *carX, const int inccarX,
+                    float *priY, const int incpriY, float *carY, const int inccarY); /* binned.h:289 */
+/* binned.h:295 -- host or device arrays (both operands must be of one kind) */
+void binned_sbsbaddv(const int fold, const int N, const float_binned *X, const int incX, float_binned *Y,
+                     const int incY);
+void binned_sbsadd(const int fold, const float X, float_binned *Y); /* binned.h:302 */
+void binned_smsadd(const int fold, const float X, float *priY, const int incpriY, float *carY,
+                   const int inccarY);                          /* binned.h:303 */
+void binned_sbsupdate(const int fold, const float X, float_binned *Y); /* binned.h:313 */
+void binned_smsupdate(const int fold, const float X, float *priY, const int incpriY, float *carY,
+                      const int inccarY);                       /* binned.h:314 */
+void binned_sbsdeposit(const int fold, const float X, float_binned *Y); /* binned.h:324 */
+void binned_smsdeposit(const int fold, const float X, float *priY, const int incpriY); /* binned.h:325 */
+void binned_sbrenorm(const int fold, float_binned *X);          /* binned.h:333 */
+void binned_smrenorm(const int fold, float *priX, const int incpriX, float *carX,
+                     const int inccarX);                        /* binned.h:334 */
+void binned_sbsconv(const int fold, const float X, float_binned *Y); /* binned.h:342 */
+void binned_smsconv(const int fold, const float X, float *priY, const int incpriY, float *carY,
+                    const int inccarY);                         /* binned.h:343 */
+float binned_ssbconv(const int fold, const float_binned *X);    /* binned.h:350 */
+float binned_ssmconv(const int fold, const float *priX, const int incpriX, const float *carX,
+                     const int inccarX);                        /* binned.h:351 */
+void binned_sbnegate(const int fold, float_binned *X);          /* binned.h:359 */
+void binned_smnegate(const int fold, float *priX, const int incpriX, float *carX,
+                     const int inccarX);                        /* binned.h:360 */
+float binned_sscale(const float X);                             /* binned.h:365 */
+void binned_smsrescale(const int fold, const float X, const float scaleY, float *priY, const int incpriY,
+                       float *carY, const int inccarY);         /* binned.h:369 */
+float binned_smsmaddsq(const int fold, const float scaleX, const float *priX, const int incpriX,
+                       const float *carX, const int inccarX, const float scaleY, float *priY,
+                       const int incpriY, float *carY, const int inccarY); /* binned.h:374 */
+float binned_sbsbaddsq(const int fold, const float scaleX, const float_binned *X, const float scaleY,
+                       float_binned *Y);                        /* binned.h:375 */
+float binned_ufpf(const float X);                               /* binned.h:378 */
 
 #ifdef __cplusplus
 }

@@ -1,10 +1,10 @@
 /*
- * binnedBLAS.h -- binned BLAS over double vectors and matrices, as exported by
- * libreproblas_hip.so.
+ * binnedBLAS.h -- binned BLAS over double and float vectors and matrices, as
+ * exported by libreproblas_hip.so.
  *
- * Drop-in for the double-precision part of the reference's
- * include/binnedBLAS.h (ReproBLAS 2.1.0). The dbd* calls accumulate into a
- * caller-owned binned value (or vector of them), as the reference's do.
+ * Drop-in for the real (double and float) part of the reference's
+ * include/binnedBLAS.h (ReproBLAS 2.1.0). The dbd* / sbs* calls accumulate
+ * into a caller-owned binned value (or vector of them), as the reference's do.
  *
  * Operands may live in device memory (hipMalloc / managed) -- then the work
  * runs in place on the GPU -- or in host memory, which is staged to the GPU
@@ -47,6 +47,34 @@ void binnedBLAS_dbdgemv(const int fold, const char Order, const char TransA, con
 void binnedBLAS_dbdgemm(const int fold, const char Order, const char TransA, const char TransB, const int M,
                         const int N, const int K, const double alpha, const double *A, const int lda,
                         const double *B, const int ldb, double_binned *C, const int ldc); /* binnedBLAS.h:92 */
+
+/* single precision (float_binned accumulators) */
+float binnedBLAS_samax(const int N, const float *X, const int incX);                /* binnedBLAS.h:37 */
+float binnedBLAS_samaxm(const int N, const float *X, const int incX, const float *Y,
+                        const int incY);                                            /* binnedBLAS.h:42 */
+void binnedBLAS_sbssum(const int fold, const int N, const float *X, const int incX,
+                       float_binned *Y);                                            /* binnedBLAS.h:67 */
+void binnedBLAS_smssum(const int fold, const int N, const float *X, const int incX, float *priY,
+                       const int incpriY, float *carY, const int inccarY);          /* binnedBLAS.h:68 */
+void binnedBLAS_sbsasum(const int fold, const int N, const float *X, const int incX,
+                        float_binned *Y);                                           /* binnedBLAS.h:69 */
+void binnedBLAS_smsasum(const int fold, const int N, const float *X, const int incX, float *priY,
+                        const int incpriY, float *carY, const int inccarY);         /* binnedBLAS.h:70 */
+float binnedBLAS_sbsssq(const int fold, const int N, const float *X, const int incX, const float scaleY,
+                        float_binned *Y);                                           /* binnedBLAS.h:71 */
+float binnedBLAS_smsssq(const int fold, const int N, const float *X, const int incX, const float scaleY,
+                        float *priY, const int incpriY, float *carY, const int inccarY); /* binnedBLAS.h:72 */
+void binnedBLAS_sbsdot(const int fold, const int N, const float *X, const int incX, const float *Y,
+                       const int incY, float_binned *Z);                            /* binnedBLAS.h:73 */
+void binnedBLAS_smsdot(const int fold, const int N, const float *X, const int incX, const float *Y,
+                       const int incY, float *manZ, const int incmanZ, float *carZ,
+                       const int inccarZ);                                          /* binnedBLAS.h:74 */
+void binnedBLAS_sbsgemv(const int fold, const char Order, const char TransA, const int M, const int N,
+                        const float alpha, const float *A, const int lda, const float *X, const int incX,
+                        float_binned *Y, const int incY);                           /* binnedBLAS.h:99 */
+void binnedBLAS_sbsgemm(const int fold, const char Order, const char TransA, const char TransB, const int M,
+                        const int N, const int K, const float alpha, const float *A, const int lda,
+                        const float *B, const int ldb, float_binned *C, const int ldc); /* binnedBLAS.h:104 */
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,9 @@ extern "C" {
 #ifndef DIDEFAULTFOLD
 #define DIDEFAULTFOLD 3 /* reference config.h:14 */
 #endif
+#ifndef SIDEFAULTFOLD
+#define SIDEFAULTFOLD 3 /* reference config.h:22 */
+#endif
 
 double reproBLAS_rdsum(const int fold, const int N, const double *X, const int incX);  /* reproBLAS.h:27 */
 double reproBLAS_rdasum(const int fold, const int N, const double *X, const int incX); /* reproBLAS.h:28 */
@@ -51,6 +54,30 @@ void reproBLAS_dgemv(const char Order, const char TransA, const int M, const int
 void reproBLAS_dgemm(const char Order, const char TransA, const char TransB, const int M, const int N, const int K,
                      const double alpha, const double *A, const int lda, const double *B, const int ldb,
                      const double beta, double *C, const int ldc); /* reproBLAS.h:124 */
+
+/* single precision (float_binned accumulators, folds 2..21) */
+float reproBLAS_rssum(const int fold, const int N, const float *X, const int incX);  /* reproBLAS.h:34 */
+float reproBLAS_rsasum(const int fold, const int N, const float *X, const int incX); /* reproBLAS.h:33 */
+float reproBLAS_rsnrm2(const int fold, const int N, const float *X, const int incX); /* reproBLAS.h:35 */
+float reproBLAS_rsdot(const int fold, const int N, const float *X, const int incX, const float *Y,
+                      const int incY);                                                /* reproBLAS.h:32 */
+void reproBLAS_rsgemv(const int fold, const char Order, const char TransA, const int M, const int N,
+                      const float alpha, const float *A, const int lda, const float *X, const int incX,
+                      const float beta, float *Y, const int incY); /* reproBLAS.h:61 */
+void reproBLAS_rsgemm(const int fold, const char Order, const char TransA, const char TransB, const int M,
+                      const int N, const int K, const float alpha, const float *A, const int lda, const float *B,
+                      const int ldb, const float beta, float *C, const int ldc); /* reproBLAS.h:66 */
+float reproBLAS_ssum(const int N, const float *X, const int incX);  /* reproBLAS.h:104 */
+float reproBLAS_sasum(const int N, const float *X, const int incX); /* reproBLAS.h:103 */
+float reproBLAS_snrm2(const int N, const float *X, const int incX); /* reproBLAS.h:105 */
+float reproBLAS_sdot(const int N, const float *X, const int incX, const float *Y,
+                     const int incY); /* reproBLAS.h:102 */
+void reproBLAS_sgemv(const char Order, const char TransA, const int M, const int N, const float alpha,
+                     const float *A, const int lda, const float *X, const int incX, const float beta, float *Y,
+                     const int incY); /* reproBLAS.h:130 */
+void reproBLAS_sgemm(const char Order, const char TransA, const char TransB, const int M, const int N, const int K,
+                     const float alpha, const float *A, const int lda, const float *B, const int ldb,
+                     const float beta, float *C, const int ldc); /* reproBLAS.h:135 */
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,26 @@ int reproblas_hip_rdgemm_device(int fold, char Order, char TransA, char TransB, 
                                 double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
                                 double beta, double *C, int64_t ldc, void *stream);
 
+/* Single-precision counterparts on device data (folds 2..21; float_binned
+ * states of 2*fold floats; reproBLAS_rs* / binnedBLAS_sbs* semantics). */
+int reproblas_hip_rssum_device(int fold, int64_t N, const float *X, int64_t incX, float *result, void *stream);
+int reproblas_hip_rsasum_device(int fold, int64_t N, const float *X, int64_t incX, float *result, void *stream);
+int reproblas_hip_rsnrm2_device(int fold, int64_t N, const float *X, int64_t incX, float *result, void *stream);
+int reproblas_hip_rsdot_device(int fold, int64_t N, const float *X, int64_t incX, const float *Y, int64_t incY,
+                               float *result, void *stream);
+int reproblas_hip_sbssum_device(int fold, int64_t N, const float *X, int64_t incX, float_binned *Y, void *stream);
+int reproblas_hip_sbsasum_device(int fold, int64_t N, const float *X, int64_t incX, float_binned *Y, void *stream);
+int reproblas_hip_sbsdot_device(int fold, int64_t N, const float *X, int64_t incX, const float *Y, int64_t incY,
+                                float_binned *Z, void *stream);
+int reproblas_hip_sbsssq_device(int fold, int64_t N, const float *X, int64_t incX, float *scale, float_binned *Y,
+                                void *stream);
+int reproblas_hip_rsgemv_device(int fold, char Order, char TransA, int64_t M, int64_t N, float alpha,
+                                const float *A, int64_t lda, const float *X, int64_t incX, float beta, float *Y,
+                                int64_t incY, void *stream);
+int reproblas_hip_rsgemm_device(int fold, char Order, char TransA, char TransB, int64_t M, int64_t N, int64_t K,
+                                float alpha, const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
+                                float *C, int64_t ldc, void *stream);
+
 /* gemm tiling (the autotuner's knob; every tiling gives identical bits).
  * Returns the number of tilings; set/get select by ordinal. */
 int reproblas_hip_gemm_tiling_count(void);

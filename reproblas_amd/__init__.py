@@ -1,4 +1,4 @@
-"""reproblas_amd -- reproducible (binned) FP64 BLAS on AMD Instinct MI355X.
+"""reproblas_amd -- reproducible (binned) FP64 and FP32 BLAS on AMD Instinct MI355X.
 
 Python mirror of the reproBLAS C API (reference include/reproBLAS.h,
 include/binnedBLAS.h) over the C ABI of ``libreproblas_hip.so``. Every
@@ -11,6 +11,7 @@ if the library is missing the import of any compute function fails loudly.
     s = rb.rdsum(x)                    # fold 3, like reproBLAS_dsum
     d = rb.rddot(x, y, fold=2)         # like reproBLAS_rddot(2, ...)
     rb.rdgemm('N', 'N', 1.0, A, B, 0.0, C)   # column-major, in place on C
+    t = rb.rssum(x32)                  # float32 input, float_binned state
 """
 import ctypes
 import os
@@ -22,11 +23,16 @@ __all__ = [
     "dbdsum", "dbdasum", "dbddot", "dbdssq", "dbdgemv", "dbdgemm", "damax", "damaxm",
     "dbdconv", "ddbconv", "dbdbadd", "dbdbaddv", "dbsetzero", "dbnum",
     "gemm_tilings", "set_gemm_tiling", "get_gemm_tiling",
+    "SDEFAULT_FOLD", "rssum", "rsasum", "rsnrm2", "rsdot", "rsgemv", "rsgemm",
+    "ssum", "sasum", "snrm2", "sdot", "sgemv", "sgemm",
+    "sbssum", "sbsasum", "sbsdot", "sbsssq", "sbsgemv", "sbsgemm", "samax", "samaxm",
+    "sbsconv", "ssbconv", "sbsbadd", "sbsbaddv", "sbsetzero", "sbnum",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libreproblas_hip.so")
 DEFAULT_FOLD = 3  # reference config.h:14 (DIDEFAULTFOLD)
+SDEFAULT_FOLD = 3  # reference config.h:22 (SIDEFAULTFOLD)
 
 
 class ReproBLASError(RuntimeError):
@@ -35,6 +41,7 @@ class ReproBLASError(RuntimeError):
 
 _lib = None
 _c_int, _c_i64, _c_double, _c_char, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_char, ctypes.c_void_p
+_c_float = ctypes.c_float
 
 _SIGS = {
     # reproBLAS.h
@@ -92,6 +99,43 @@ _SIGS = {
                                              _c_i64, _c_double, _vp, _c_i64, _vp]),
     "reproblas_hip_rdgemm_device": (_c_int, [_c_int, _c_char, _c_char, _c_char, _c_i64, _c_i64, _c_i64, _c_double,
                                              _vp, _c_i64, _vp, _c_i64, _c_double, _vp, _c_i64, _vp]),
+    # single precision (reproBLAS.h, binnedBLAS.h, binned.h, reproblas_hip.h)
+    "reproBLAS_rssum": (_c_float, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rsasum": (_c_float, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rsnrm2": (_c_float, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rsdot": (_c_float, [_c_int, _c_int, _vp, _c_int, _vp, _c_int]),
+    "reproBLAS_rsgemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _c_float, _vp, _c_int, _vp, _c_int,
+                                _c_float, _vp, _c_int]),
+    "reproBLAS_rsgemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _c_float, _vp, _c_int,
+                                _vp, _c_int, _c_float, _vp, _c_int]),
+    "binnedBLAS_samax": (_c_float, [_c_int, _vp, _c_int]),
+    "binnedBLAS_samaxm": (_c_float, [_c_int, _vp, _c_int, _vp, _c_int]),
+    "binnedBLAS_sbssum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
+    "binnedBLAS_sbsasum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
+    "binnedBLAS_sbsdot": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp]),
+    "binnedBLAS_sbsssq": (_c_float, [_c_int, _c_int, _vp, _c_int, _c_float, _vp]),
+    "binnedBLAS_sbsgemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _c_float, _vp, _c_int, _vp, _c_int,
+                                  _vp, _c_int]),
+    "binnedBLAS_sbsgemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _c_float, _vp, _c_int,
+                                  _vp, _c_int, _vp, _c_int]),
+    "binned_sbnum": (_c_int, [_c_int]),
+    "binned_sbsetzero": (None, [_c_int, _vp]),
+    "binned_sbsconv": (None, [_c_int, _c_float, _vp]),
+    "binned_ssbconv": (_c_float, [_c_int, _vp]),
+    "binned_sbsbadd": (None, [_c_int, _vp, _vp]),
+    "binned_sbsbaddv": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int]),
+    "reproblas_hip_rssum_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_rsasum_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_rsnrm2_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_rsdot_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_sbssum_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_sbsasum_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_sbsdot_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "reproblas_hip_sbsssq_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp, _vp]),
+    "reproblas_hip_rsgemv_device": (_c_int, [_c_int, _c_char, _c_char, _c_i64, _c_i64, _c_float, _vp, _c_i64, _vp,
+                                             _c_i64, _c_float, _vp, _c_i64, _vp]),
+    "reproblas_hip_rsgemm_device": (_c_int, [_c_int, _c_char, _c_char, _c_char, _c_i64, _c_i64, _c_i64, _c_float,
+                                             _vp, _c_i64, _vp, _c_i64, _c_float, _vp, _c_i64, _vp]),
     "reproblas_hip_gemm_tiling_count": (_c_int, []),
     "reproblas_hip_gemm_set_tiling": (_c_int, [_c_int]),
     "reproblas_hip_gemm_get_tiling": (_c_int, []),
@@ -119,19 +163,19 @@ def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
 
-def _ptr(a):
-    """Address of the first element of a numpy array or torch tensor."""
+def _ptr(a, dtype="float64"):
+    """Address of the first element of a numpy array or torch tensor of ``dtype``."""
     if a is None:
         return None
     if _is_torch(a):
-        if str(a.dtype) != "torch.float64":
-            raise TypeError("reproblas_amd works on float64 tensors")
+        if str(a.dtype) != "torch." + dtype:
+            raise TypeError(f"this call works on {dtype} tensors, got {a.dtype}")
         if not a.is_contiguous():
             raise ValueError("reproblas_amd needs contiguous tensors (pass strides through inc/ld arguments)")
         return ctypes.c_void_p(a.data_ptr())
     import numpy as np
-    if not isinstance(a, np.ndarray) or a.dtype != np.float64:
-        raise TypeError("reproblas_amd works on float64 numpy arrays or torch tensors")
+    if not isinstance(a, np.ndarray) or a.dtype != np.dtype(dtype):
+        raise TypeError(f"this call works on {dtype} numpy arrays or torch tensors")
     if not a.flags["C_CONTIGUOUS"]:
         raise ValueError("reproblas_amd needs contiguous arrays (pass strides through inc/ld arguments)")
     return ctypes.c_void_p(a.ctypes.data)
@@ -368,3 +412,187 @@ def set_gemm_tiling(t):
 
 def get_gemm_tiling():
     return lib().reproblas_hip_gemm_get_tiling()
+
+
+# ===================================================== single precision --
+# float32 inputs, float_binned states of 2*fold float32 (folds 2..21);
+# reference include/reproBLAS.h:32-35,61,66 and binnedBLAS.h:37-104.
+def _sp(a):
+    return _ptr(a, "float32")
+
+
+def rssum(x, fold=SDEFAULT_FOLD, inc=1, n=None):
+    """reproBLAS_rssum: reproducible float sum."""
+    _bind_stream(x)
+    r = lib().reproBLAS_rssum(fold, _n_of(x, inc) if n is None else n, _sp(x), inc)
+    _check()
+    return r
+
+
+def rsasum(x, fold=SDEFAULT_FOLD, inc=1, n=None):
+    _bind_stream(x)
+    r = lib().reproBLAS_rsasum(fold, _n_of(x, inc) if n is None else n, _sp(x), inc)
+    _check()
+    return r
+
+
+def rsnrm2(x, fold=SDEFAULT_FOLD, inc=1, n=None):
+    _bind_stream(x)
+    r = lib().reproBLAS_rsnrm2(fold, _n_of(x, inc) if n is None else n, _sp(x), inc)
+    _check()
+    return r
+
+
+def rsdot(x, y, fold=SDEFAULT_FOLD, incx=1, incy=1, n=None):
+    _bind_stream(x, y)
+    if n is None:
+        n = min(_n_of(x, incx), _n_of(y, incy))
+    r = lib().reproBLAS_rsdot(fold, n, _sp(x), incx, _sp(y), incy)
+    _check()
+    return r
+
+
+def ssum(x, inc=1):
+    return rssum(x, SDEFAULT_FOLD, inc)
+
+
+def sasum(x, inc=1):
+    return rsasum(x, SDEFAULT_FOLD, inc)
+
+
+def snrm2(x, inc=1):
+    return rsnrm2(x, SDEFAULT_FOLD, inc)
+
+
+def sdot(x, y, incx=1, incy=1):
+    return rsdot(x, y, SDEFAULT_FOLD, incx, incy)
+
+
+def sbnum(fold=SDEFAULT_FOLD):
+    return 2 * fold
+
+
+def sbssum(Y, x, fold=SDEFAULT_FOLD, inc=1):
+    """binnedBLAS_sbssum: Y (a 2*fold float32 binned value) += sum(x)."""
+    _bind_stream(x, Y)
+    lib().binnedBLAS_sbssum(fold, _n_of(x, inc), _sp(x), inc, _sp(Y))
+    _check()
+
+
+def sbsasum(Y, x, fold=SDEFAULT_FOLD, inc=1):
+    _bind_stream(x, Y)
+    lib().binnedBLAS_sbsasum(fold, _n_of(x, inc), _sp(x), inc, _sp(Y))
+    _check()
+
+
+def sbsdot(Z, x, y, fold=SDEFAULT_FOLD, incx=1, incy=1):
+    _bind_stream(x, y, Z)
+    n = min(_n_of(x, incx), _n_of(y, incy))
+    lib().binnedBLAS_sbsdot(fold, n, _sp(x), incx, _sp(y), incy, _sp(Z))
+    _check()
+
+
+def sbsssq(Y, x, scale=0.0, fold=SDEFAULT_FOLD, inc=1):
+    """binnedBLAS_sbsssq: returns the new scale; Y accumulates the scaled squares."""
+    _bind_stream(x, Y)
+    r = lib().binnedBLAS_sbsssq(fold, _n_of(x, inc), _sp(x), inc, scale, _sp(Y))
+    _check()
+    return r
+
+
+def samax(x, inc=1):
+    _bind_stream(x)
+    r = lib().binnedBLAS_samax(_n_of(x, inc), _sp(x), inc)
+    _check()
+    return r
+
+
+def samaxm(x, y, incx=1, incy=1):
+    _bind_stream(x, y)
+    r = lib().binnedBLAS_samaxm(min(_n_of(x, incx), _n_of(y, incy)), _sp(x), incx, _sp(y), incy)
+    _check()
+    return r
+
+
+def sbsetzero(Y, fold=SDEFAULT_FOLD):
+    lib().binned_sbsetzero(fold, _sp(Y))
+
+
+def sbsconv(x, fold=SDEFAULT_FOLD):
+    import numpy as np
+    Y = np.zeros(2 * fold, np.float32)
+    lib().binned_sbsconv(fold, x, _sp(Y))
+    return Y
+
+
+def ssbconv(Y, fold=SDEFAULT_FOLD):
+    return lib().binned_ssbconv(fold, _sp(Y))
+
+
+def sbsbadd(X, Y, fold=SDEFAULT_FOLD):
+    lib().binned_sbsbadd(fold, _sp(X), _sp(Y))
+
+
+def sbsbaddv(X, Y, n, fold=SDEFAULT_FOLD, incx=1, incy=1):
+    _bind_stream(X, Y)
+    lib().binned_sbsbaddv(fold, n, _sp(X), incx, _sp(Y), incy)
+    _check()
+
+
+def rsgemv(trans, alpha, A, x, beta, y, fold=SDEFAULT_FOLD, order="C", M=None, N=None, lda=None, incx=1, incy=1):
+    """reproBLAS_rsgemv on a 2-D float32 A (shapes as for :func:`rdgemv`)."""
+    rm = order in ("r", "R")
+    if M is None or N is None or lda is None:
+        sh = tuple(A.shape)
+        if rm:
+            M, N, lda = sh[0], sh[1], sh[1]
+        else:
+            M, N, lda = sh[1], sh[0], sh[1]
+    _bind_stream(A, x, y)
+    lib().reproBLAS_rsgemv(fold, _ch(order), _ch(trans), M, N, alpha, _sp(A), lda, _sp(x), incx, beta, _sp(y),
+                           incy)
+    _check()
+
+
+def rsgemm(transa, transb, alpha, A, B, beta, C, fold=SDEFAULT_FOLD, order="C", M=None, N=None, K=None,
+           lda=None, ldb=None, ldc=None):
+    """reproBLAS_rsgemm on float32 operands; C is updated in place."""
+    rm = order in ("r", "R")
+
+    def dims(X):
+        s = tuple(X.shape)
+        return (s[0], s[1], s[1]) if rm else (s[1], s[0], s[1])
+
+    ra, ca, la = dims(A)
+    rb_, cb, lb = dims(B)
+    rc, cc, lc = dims(C)
+    na = transa in ("n", "N")
+    M = (ra if na else ca) if M is None else M
+    K = (ca if na else ra) if K is None else K
+    N = (cb if transb in ("n", "N") else rb_) if N is None else N
+    lda, ldb, ldc = lda or la, ldb or lb, ldc or lc
+    _bind_stream(A, B, C)
+    lib().reproBLAS_rsgemm(fold, _ch(order), _ch(transa), _ch(transb), M, N, K, alpha, _sp(A), lda, _sp(B), ldb,
+                           beta, _sp(C), ldc)
+    _check()
+
+
+def sgemv(trans, alpha, A, x, beta, y, **kw):
+    return rsgemv(trans, alpha, A, x, beta, y, fold=SDEFAULT_FOLD, **kw)
+
+
+def sgemm(transa, transb, alpha, A, B, beta, C, **kw):
+    return rsgemm(transa, transb, alpha, A, B, beta, C, fold=SDEFAULT_FOLD, **kw)
+
+
+def sbsgemv(fold, order, trans, M, N, alpha, A, lda, x, incx, Yb, incy=1):
+    _bind_stream(A, x, Yb)
+    lib().binnedBLAS_sbsgemv(fold, _ch(order), _ch(trans), M, N, alpha, _sp(A), lda, _sp(x), incx, _sp(Yb), incy)
+    _check()
+
+
+def sbsgemm(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, Cb, ldc):
+    _bind_stream(A, B, Cb)
+    lib().binnedBLAS_sbsgemm(fold, _ch(order), _ch(transa), _ch(transb), M, N, K, alpha, _sp(A), lda, _sp(B), ldb,
+                             _sp(Cb), ldc)
+    _check()

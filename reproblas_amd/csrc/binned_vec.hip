@@ -1,8 +1,10 @@
-// binned_vec.hip -- elementwise kernels over device arrays of binned values:
-// the vector add behind binned_dbdbaddv (reference src/binned/dbdbaddv.c:18-23),
-// double -> binned (binned_dbdconv, src/binned/dmdconv.c:18-21, with the beta
-// handling of src/reproBLAS/rdgemv.c:50-61 and rdgemm.c:80-95) and binned ->
-// double (binned_ddbconv, src/binned/ddmconv.c:23-81).
+// binned_vec.hip -- elementwise kernels over device arrays of binned values
+// (double_binned and float_binned): the vector add behind binned_dbdbaddv /
+// binned_sbsbaddv (reference src/binned/dbdbaddv.c:18-23, sbsbaddv.c),
+// scalar -> binned (binned_dbdconv / sbsconv, src/binned/dmdconv.c:18-21,
+// with the beta handling of src/reproBLAS/rdgemv.c:50-61 and rdgemm.c:80-95)
+// and binned -> scalar (binned_ddbconv / ssbconv, src/binned/ddmconv.c:23-81,
+// ssmconv.c).
 #include "binned_core.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -11,15 +13,15 @@ namespace rb {
 
 namespace {
 
-template <int CAP, bool FIXED>
-__global__ __launch_bounds__(256) void addv_kernel(int fold_rt, int64_t n, const double* X, int64_t incX, double* Y,
+template <int CAP, bool FIXED, class F>
+__global__ __launch_bounds__(256) void addv_kernel(int fold_rt, int64_t n, const F* X, int64_t incX, F* Y,
                                                    int64_t incY) {
   const int fold = FIXED ? CAP : fold_rt;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double* x = X + i * incX * 2 * fold;
-  double* y = Y + i * incY * 2 * fold;
-  double px[CAP], cx[CAP], py[CAP], cy[CAP];
+  const F* x = X + i * incX * 2 * fold;
+  F* y = Y + i * incY * 2 * fold;
+  F px[CAP], cx[CAP], py[CAP], cy[CAP];
 RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
@@ -37,16 +39,16 @@ RB_UNROLL_CAP
     }
 }
 
-template <int CAP, bool FIXED>
-__global__ __launch_bounds__(256) void to_binned_kernel(int fold_rt, int64_t n, const double* v, int64_t incv,
-                                                        double beta, double* Yb) {
+template <int CAP, bool FIXED, class F>
+__global__ __launch_bounds__(256) void to_binned_kernel(int fold_rt, int64_t n, const F* v, int64_t incv, F beta,
+                                                        F* Yb) {
   const int fold = FIXED ? CAP : fold_rt;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double p[CAP], c[CAP];
+  F p[CAP], c[CAP];
   set_zero(fold, p, c);
-  if (beta != 0.0) {
-    const double x = (beta == 1.0) ? v[i * incv] : v[i * incv] * beta;
+  if (beta != F(0)) {
+    const F x = (beta == F(1)) ? v[i * incv] : v[i * incv] * beta;
     // binned_dmdadd: update, deposit (with its NaN/Inf shortcut), renorm
     update(fold, x, p, c);
     if (nan_or_inf(x) || nan_or_inf(p[0])) {
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(256) void to_binned_kernel(int fold_rt, int64_t n, 
     }
     renorm(fold, p, c);
   }
-  double* y = Yb + i * 2 * fold;
+  F* y = Yb + i * 2 * fold;
 RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
@@ -65,68 +67,90 @@ RB_UNROLL_CAP
     }
 }
 
-template <int CAP, bool FIXED>
-__global__ __launch_bounds__(256) void to_double_kernel(int fold_rt, int64_t n, const double* Yb, double* v,
-                                                        int64_t incv) {
+template <int CAP, bool FIXED, class F>
+__global__ __launch_bounds__(256) void from_binned_kernel(int fold_rt, int64_t n, const F* Yb, F* v, int64_t incv) {
   const int fold = FIXED ? CAP : fold_rt;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double* y = Yb + i * 2 * fold;
-  double p[CAP], c[CAP];
+  const F* y = Yb + i * 2 * fold;
+  F p[CAP], c[CAP];
 RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
       p[j] = y[j];
       c[j] = y[fold + j];
     }
-  v[i * incv] = to_double(fold, p, c);
+  v[i * incv] = convert(fold, p, c);
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+template <class F>
+bool fold_ok(int fold) {
+  if (fold >= 2 && fold <= Fmt<F>::max_fold) return true;
+  set_error(kBadFold, "fold %d is outside the supported range [2, %d]", fold, Fmt<F>::max_fold);
+  return false;
+}
+
 }  // namespace
 
-#define RB_FOLD_DISPATCH(KERNEL, ...)                                                         \
+#define RB_FOLD_DISPATCH(KERNEL, F, ...)                                                      \
   switch (fold) {                                                                             \
-    case 2: KERNEL<2, true><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;            \
-    case 3: KERNEL<3, true><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;            \
-    case 4: KERNEL<4, true><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;            \
+    case 2: KERNEL<2, true, F><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;         \
+    case 3: KERNEL<3, true, F><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;         \
+    case 4: KERNEL<4, true, F><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__); break;         \
     default:                                                                                  \
       if (fold <= 8)                                                                          \
-        KERNEL<8, false><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__);                      \
+        KERNEL<8, false, F><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__);                   \
       else                                                                                    \
-        KERNEL<kMaxFold, false><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__);               \
+        KERNEL<Fmt<F>::max_fold, false, F><<<grid_for(n), 256, 0, s>>>(fold, __VA_ARGS__);    \
       break;                                                                                  \
   }
 
 bool dbdbaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int64_t incY, hipStream_t s) {
-  if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
-    return false;
-  }
+  if (!fold_ok<double>(fold)) return false;
   if (n <= 0) return true;
-  RB_FOLD_DISPATCH(addv_kernel, n, X, incX, Y, incY);
+  RB_FOLD_DISPATCH(addv_kernel, double, n, X, incX, Y, incY);
+  return RB_HIP(hipGetLastError());
+}
+
+bool sbsbaddv(int fold, int64_t n, const float* X, int64_t incX, float* Y, int64_t incY, hipStream_t s) {
+  if (!fold_ok<float>(fold)) return false;
+  if (n <= 0) return true;
+  RB_FOLD_DISPATCH(addv_kernel, float, n, X, incX, Y, incY);
   return RB_HIP(hipGetLastError());
 }
 
 bool conv_to_binned(int fold, int64_t n, const double* y, int64_t incy, double beta, double* Yb, hipStream_t s) {
-  if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
-    return false;
-  }
+  if (!fold_ok<double>(fold)) return false;
   if (n <= 0) return true;
-  RB_FOLD_DISPATCH(to_binned_kernel, n, y, incy, beta, Yb);
+  RB_FOLD_DISPATCH(to_binned_kernel, double, n, y, incy, beta, Yb);
+  return RB_HIP(hipGetLastError());
+}
+
+bool conv_to_binned(int fold, int64_t n, const float* y, int64_t incy, float beta, float* Yb, hipStream_t s) {
+  if (!fold_ok<float>(fold)) return false;
+  if (n <= 0) return true;
+  RB_FOLD_DISPATCH(to_binned_kernel, float, n, y, incy, beta, Yb);
+  return RB_HIP(hipGetLastError());
+}
+
+bool conv_from_binned(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s) {
+  if (!fold_ok<double>(fold)) return false;
+  if (n <= 0) return true;
+  RB_FOLD_DISPATCH(from_binned_kernel, double, n, Yb, y, incy);
+  return RB_HIP(hipGetLastError());
+}
+
+bool conv_from_binned(int fold, int64_t n, const float* Yb, float* y, int64_t incy, hipStream_t s) {
+  if (!fold_ok<float>(fold)) return false;
+  if (n <= 0) return true;
+  RB_FOLD_DISPATCH(from_binned_kernel, float, n, Yb, y, incy);
   return RB_HIP(hipGetLastError());
 }
 
 bool conv_to_double(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s) {
-  if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
-    return false;
-  }
-  if (n <= 0) return true;
-  RB_FOLD_DISPATCH(to_double_kernel, n, Yb, y, incy);
-  return RB_HIP(hipGetLastError());
+  return conv_from_binned(fold, n, Yb, y, incy, s);
 }
 
 }  // namespace rb

@@ -33,41 +33,42 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 
+template <class F>
 struct Blas1Args {
   int64_t n;
-  const double* x;
+  const F* x;
   int64_t incx;
-  const double* y;
+  const F* y;
   int64_t incy;
-  double* partials;    // gridDim.x records of (2*fold+1) doubles
+  F* partials;    // gridDim.x records of (2*fold+1) elements
   unsigned* ticket;
-  const double* state_in;
-  double scale_in;
-  const double* scale_in_dev;  // when set, the scale of state_in is read here
-  double* state_out;
-  double* scale_out;
-  double* result;
+  const F* state_in;
+  F scale_in;
+  const F* scale_in_dev;  // when set, the scale of state_in is read here
+  F* state_out;
+  F* scale_out;
+  F* result;
   int fold;
 };
 
-template <int CAP, bool FIXED, int T, int U, int MODE>
-__global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
+template <int CAP, bool FIXED, int T, int U, int MODE, class F>
+__global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar)
   const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + wave;
   const int64_t nwaves = (int64_t)gridDim.x * kWaves;
   const Span sp = make_span<U>(a.x, a.incx, a.y, a.incy, a.n);
-  State<CAP> s;
+  State<CAP, F> s;
   wave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s);
 
   // Workgroup merge through LDS, then one partial per workgroup.
-  __shared__ double lds[kWaves][2 * CAP + 1];
+  __shared__ F lds[kWaves][2 * CAP + 1];
   __shared__ bool last;
   if (lane == 0) store_state(lds[wave], fold, s);
   __syncthreads();
   if (threadIdx.x == 0) {
-    State<CAP> acc, o;
+    State<CAP, F> acc, o;
     load_state(lds[0], fold, acc);
     for (int w = 1; w < kWaves; ++w) {
       load_state(lds[w], fold, o);
@@ -84,11 +85,11 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args a) {
   // Last workgroup: merge every partial (exact, so the order is free; it is
   // fixed anyway), fold in the caller's value, convert.
   __threadfence();
-  State<CAP> acc;
+  State<CAP, F> acc;
   bool have = false;
   for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
-    State<CAP> o;
-    const double* src = a.partials + (int64_t)b * (2 * fold + 1);
+    State<CAP, F> o;
+    const F* src = a.partials + (int64_t)b * (2 * fold + 1);
 RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j)
       if (j < fold) {
@@ -111,14 +112,14 @@ RB_UNROLL_CAP
   if (lane == 0) store_state(lds[wave], fold, acc);
   __syncthreads();
   if (threadIdx.x == 0) {
-    State<CAP> o;
+    State<CAP, F> o;
     load_state(lds[0], fold, acc);
     for (int w = 1; w < kWaves; ++w) {
       load_state(lds[w], fold, o);
       merge<CAP, T>(fold, acc, o);
     }
     // caller's value, renormalised first (the reference ends with a renorm)
-    State<CAP> y;
+    State<CAP, F> y;
     if (a.state_in) {
 RB_UNROLL_CAP
       for (int j = 0; j < CAP; ++j)
@@ -139,8 +140,8 @@ RB_UNROLL_CAP
       merge<CAP, T>(fold, y, acc);
     }
     renorm(fold, y.p, y.c);
-    double scale = y.scl;
-    if (T == kSSq && nan_or_inf(y.p[0])) scale = 1.0;  // dscale(1.0), dmdssq.ccog tail
+    F scale = y.scl;
+    if (T == kSSq && nan_or_inf(y.p[0])) scale = F(1);  // dscale(1.0), dmdssq.ccog tail
     if (a.state_out) {
 RB_UNROLL_CAP
       for (int j = 0; j < CAP; ++j)
@@ -151,25 +152,26 @@ RB_UNROLL_CAP
     }
     if (a.scale_out) *a.scale_out = scale;
     if (a.result) {
-      const double v = to_double(fold, y.p, y.c);
-      *a.result = (T == kSSq) ? scale * sqrt(v) : v;
+      const F v = convert(fold, y.p, y.c);
+      // nrm2: scl * sqrt(conv) in double, as rdnrm2.c:31 and rsnrm2.c compute it
+      *a.result = (T == kSSq) ? (F)((double)scale * sqrt((double)v)) : v;
     }
     *a.ticket = 0u;  // ready for the next launch on this stream
   }
 }
 
-template <int CAP, bool FIXED, int T, int U>
-bool launch_mode(int mode, const Blas1Args& a, int blocks, hipStream_t s) {
+template <int CAP, bool FIXED, int T, int U, class F>
+bool launch_mode(int mode, const Blas1Args<F>& a, int blocks, hipStream_t s) {
   switch (mode) {
-    case 0: blas1_kernel<CAP, FIXED, T, U, 0><<<blocks, kThreads, 0, s>>>(a); break;
-    case 1: blas1_kernel<CAP, FIXED, T, U, 1><<<blocks, kThreads, 0, s>>>(a); break;
-    default: blas1_kernel<CAP, FIXED, T, U, 2><<<blocks, kThreads, 0, s>>>(a); break;
+    case 0: blas1_kernel<CAP, FIXED, T, U, 0, F><<<blocks, kThreads, 0, s>>>(a); break;
+    case 1: blas1_kernel<CAP, FIXED, T, U, 1, F><<<blocks, kThreads, 0, s>>>(a); break;
+    default: blas1_kernel<CAP, FIXED, T, U, 2, F><<<blocks, kThreads, 0, s>>>(a); break;
   }
   return RB_HIP(hipGetLastError());
 }
 
-// Terms per lane per tile: 16 for the one-operand reductions, 8 for dot, so
-// every tile moves 8 KiB per wave.
+// Terms per lane per tile (double): 16 for the one-operand reductions, 8 for
+// dot, so every tile moves 8 KiB per wave; float tiles hold twice as many.
 //
 // Grid: at most kBlocksPerCU workgroups (4 waves) per CU for unit stride. On
 // MI355X a 2^28-term sum streams at ~6.7 TB/s with one workgroup per CU and
@@ -201,13 +203,19 @@ constexpr int kUDot = RB_BLAS1_U_DOT;
 constexpr int kBlocksPerCU = RB_BLAS1_BLOCKS_PER_CU;
 constexpr int kBlocksPerCUStrided = RB_BLAS1_BLOCKS_PER_CU_STRIDED;
 
-template <int CAP, bool FIXED>
-bool launch_term(Term term, int mode, const Blas1Args& a, int blocks, hipStream_t s) {
+// Terms per lane per tile for element type F: 8 KiB per wave tile.
+template <class F>
+constexpr int u_one() { return kUOne * (int)(sizeof(double) / sizeof(F)); }
+template <class F>
+constexpr int u_dot() { return kUDot * (int)(sizeof(double) / sizeof(F)); }
+
+template <int CAP, bool FIXED, class F>
+bool launch_term(Term term, int mode, const Blas1Args<F>& a, int blocks, hipStream_t s) {
   switch (term) {
-    case kSum: return launch_mode<CAP, FIXED, kSum, kUOne>(mode, a, blocks, s);
-    case kASum: return launch_mode<CAP, FIXED, kASum, kUOne>(mode, a, blocks, s);
-    case kDot: return launch_mode<CAP, FIXED, kDot, kUDot>(mode, a, blocks, s);
-    default: return launch_mode<CAP, FIXED, kSSq, kUOne>(mode, a, blocks, s);
+    case kSum: return launch_mode<CAP, FIXED, kSum, u_one<F>()>(mode, a, blocks, s);
+    case kASum: return launch_mode<CAP, FIXED, kASum, u_one<F>()>(mode, a, blocks, s);
+    case kDot: return launch_mode<CAP, FIXED, kDot, u_dot<F>()>(mode, a, blocks, s);
+    default: return launch_mode<CAP, FIXED, kSSq, u_one<F>()>(mode, a, blocks, s);
   }
 }
 
@@ -298,22 +306,24 @@ bool launch_seg_term(Term term, const SegArgs& a, int grid, hipStream_t s) {
 
 bool fold_supported(int fold) { return fold >= 2 && fold <= kMaxFold; }
 
-bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
-           const double* state_in, double scale_in, const double* scale_in_dev, double* state_out, double* scale_out,
-           double* result, hipStream_t s) {
-  if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
+namespace {
+template <class F>
+bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const F* y, int64_t incy,
+                const F* state_in, F scale_in, const F* scale_in_dev, F* state_out, F* scale_out, F* result,
+                hipStream_t s) {
+  if (fold < 2 || fold > Fmt<F>::max_fold) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, %d]", fold, Fmt<F>::max_fold);
     return false;
   }
   const DeviceInfo* dev = device_info();
   if (!dev) return false;
-  Blas1Args a;
+  Blas1Args<F> a;
   a.n = n < 0 ? 0 : n;
   a.x = x;
   a.incx = incx;
   a.y = y;
   a.incy = incy;
-  const int64_t per_tile = 64 * (term == kDot ? kUDot : kUOne);
+  const int64_t per_tile = 64 * (term == kDot ? u_dot<F>() : u_one<F>());
   const int64_t tiles = (a.n + per_tile - 1) / per_tile;
   a.state_in = state_in;
   a.scale_in = scale_in;
@@ -338,7 +348,7 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
   const int64_t cap = (int64_t)dev->cus * (mode == 2 ? kBlocksPerCUStrided : kBlocksPerCU);
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  a.partials = (double*)workspace(s, kSlotPartials, sizeof(double) * (size_t)blocks * (2 * fold + 1));
+  a.partials = (F*)workspace(s, kSlotPartials, sizeof(F) * (size_t)blocks * (2 * fold + 1));
   a.ticket = tickets(s, 1);
   if (!a.partials || !a.ticket) return false;
   switch (fold) {
@@ -347,8 +357,23 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
     case 4: return launch_term<4, true>(term, mode, a, (int)blocks, s);
     default:
       if (fold <= 8) return launch_term<8, false>(term, mode, a, (int)blocks, s);
-      return launch_term<kMaxFold, false>(term, mode, a, (int)blocks, s);
+      return launch_term<Fmt<F>::max_fold, false>(term, mode, a, (int)blocks, s);
   }
+}
+}  // namespace
+
+bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
+           const double* state_in, double scale_in, const double* scale_in_dev, double* state_out, double* scale_out,
+           double* result, hipStream_t s) {
+  return blas1_impl<double>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out, scale_out,
+                            result, s);
+}
+
+bool blas1_s(int fold, Term term, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy,
+             const float* state_in, float scale_in, const float* scale_in_dev, float* state_out, float* scale_out,
+             float* result, hipStream_t s) {
+  return blas1_impl<float>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out, scale_out,
+                           result, s);
 }
 
 bool segments(int fold, Term term, const double* x, const double* y, const SegChunk* chunks, int64_t nchunks,

@@ -40,31 +40,38 @@ constexpr int KC = 32;
 
 
 // ---- int-carry variants of renorm / raise (carries are small integers) ----
-template <int CAP>
-__device__ __forceinline__ void renorm_ic(int fold, double (&p)[CAP], int (&c)[CAP]) {
-  if (p[0] == 0.0 || nan_or_inf(p[0])) return;
+template <int CAP, class F>
+__device__ __forceinline__ void renorm_ic(int fold, F (&p)[CAP], int (&c)[CAP]) {
+  if (p[0] == F(0) || nan_or_inf(p[0])) return;
 RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) {
     if (j < fold) {
-      // bits 51:50 of the mantissa live in bits 19:18 of the high word
-      uint32_t hi = (uint32_t)(bits(p[j]) >> 32);
-      c[j] += (int)((hi >> 18) & 3u) - 2;
-      hi = (hi & ~(1u << 18)) | (1u << 19);
-      p[j] = from_bits(((uint64_t)hi << 32) | (bits(p[j]) & 0xffffffffull));
+      if constexpr (sizeof(F) == 8) {
+        // bits 51:50 of the mantissa live in bits 19:18 of the high word
+        uint32_t hi = (uint32_t)(bits(p[j]) >> 32);
+        c[j] += (int)((hi >> 18) & 3u) - 2;
+        hi = (hi & ~(1u << 18)) | (1u << 19);
+        p[j] = from_bits(((uint64_t)hi << 32) | (bits(p[j]) & 0xffffffffull));
+      } else {
+        uint32_t u = bits(p[j]);  // float: bits 22:21
+        c[j] += (int)((u >> 21) & 3u) - 2;
+        u = (u & ~(1u << 21)) | (1u << 22);
+        p[j] = fp_from_bits<float>(u);
+      }
     }
   }
 }
 
-template <int CAP>
-__device__ __forceinline__ void raise_ic(int fold, int shift, int I, double (&p)[CAP], int (&c)[CAP]) {
+template <int CAP, class F>
+__device__ __forceinline__ void raise_ic(int fold, int shift, int I, F (&p)[CAP], int (&c)[CAP]) {
   if (shift <= 0) return;
-  double np[CAP];
+  F np[CAP];
   int nc[CAP];
 RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j) {
     if (j < fold) {
       if (j >= shift) {
-        double v = 0.0;
+        F v = F(0);
         int w = 0;
 RB_UNROLL_CAP
         for (int t = 0; t < CAP; ++t)
@@ -72,7 +79,7 @@ RB_UNROLL_CAP
         np[j] = v;
         nc[j] = w;
       } else {
-        np[j] = bin(I + j);
+        np[j] = bin<F>(I + j);
         nc[j] = 0;
       }
     }
@@ -84,24 +91,25 @@ RB_UNROLL_CAP
 
 // ---------------------------------------------------------------- prepass --
 // rowmax[ch][i] = max_{k in chunk ch} |A(i,k)|, colmax[ch][j] likewise for B.
-__global__ __launch_bounds__(256) void chunk_max_kernel(int64_t rows, int64_t K, const double* X, int64_t s_row,
-                                                        int64_t s_k, double* out) {
+template <class F>
+__global__ __launch_bounds__(256) void chunk_max_kernel(int64_t rows, int64_t K, const F* X, int64_t s_row,
+                                                        int64_t s_k, F* out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t chunks = (K + KC - 1) / KC;
   if (t >= rows * chunks) return;
   const int64_t r = t % rows, ch = t / rows;
   const int64_t k0 = ch * KC;
   const int64_t k1 = (k0 + KC < K) ? k0 + KC : K;
-  double m = 0.0;
+  F m = F(0);
   for (int64_t k = k0; k < k1; ++k) m = fmax(m, fabs(X[r * s_row + k * s_k]));
   out[ch * rows + r] = m;
 }
 
-template <int CAP>
-__device__ __forceinline__ void beta_init(int fold, double beta, double yv, double (&p)[CAP], double (&c)[CAP]) {
+template <int CAP, class F>
+__device__ __forceinline__ void beta_init(int fold, F beta, F yv, F (&p)[CAP], F (&c)[CAP]) {
   set_zero(fold, p, c);
-  if (beta == 0.0) return;
-  const double v = (beta == 1.0) ? yv : yv * beta;
+  if (beta == F(0)) return;
+  const F v = (beta == F(1)) ? yv : yv * beta;
   update(fold, v, p, c);
   if (nan_or_inf(v) || nan_or_inf(p[0])) {
     p[0] += v;
@@ -112,18 +120,21 @@ __device__ __forceinline__ void beta_init(int fold, double beta, double yv, doub
 }
 
 // ------------------------------------------------------------------ kernel --
-constexpr int kPad = 2;  // keeps LDS rows 16-byte aligned, spreads staging writes over banks
+// Row padding of the LDS tiles: keeps rows 16-byte aligned and spreads the
+// staging writes over banks.
+template <class F>
+constexpr int pad() { return 16 / (int)sizeof(F); }
 
 // One (16*TM) x (16*TN) tile of C at (i0, j0), all of K.
-template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK>
-__device__ __forceinline__ void gemm_tile(const GemmDesc& g, int64_t i0, int64_t j0,
-                                          double (&As)[KC][16 * TM + kPad], double (&Bs)[KC][16 * TN + kPad]) {
+template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK, class F>
+__device__ __forceinline__ void gemm_tile(const GemmDescT<F>& g, int64_t i0, int64_t j0,
+                                          F (&As)[KC][16 * TM + pad<F>()], F (&Bs)[KC][16 * TN + pad<F>()]) {
   constexpr int BM = 16 * TM, BN = 16 * TN;
   const int fold = FIXED ? CAP : g.fold;
   const int tid = threadIdx.x;
   const int tr = tid / 16, tc = tid % 16;
 
-  double p[TM][TN][CAP];
+  F p[TM][TN][CAP];
   int c[TM][TN][CAP];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -131,13 +142,13 @@ __device__ __forceinline__ void gemm_tile(const GemmDesc& g, int64_t i0, int64_t
     for (int j = 0; j < TN; ++j)
 RB_UNROLL_CAP
       for (int f = 0; f < CAP; ++f) {
-        p[i][j][f] = bin(kMaxIndex + f);
+        p[i][j][f] = bin<F>(Fmt<F>::max_index + f);
         c[i][j][f] = 0;
       }
-  double lmin = admit_limit(kMaxIndex);  // smallest admissible limit over the lane's outputs
+  F lmin = admit_limit<F>(Fmt<F>::max_index);  // smallest admissible limit over the lane's outputs
   bool anytop = false;                   // some output of the lane at index 0
   int dep = 0;
-  const double aalpha = fabs(g.alpha);
+  const F aalpha = fabs(g.alpha);
   const int64_t chunks = (g.K + KC - 1) / KC;
 
   for (int64_t ch = 0; ch < chunks; ++ch) {
@@ -154,8 +165,8 @@ RB_UNROLL_CAP
         li = tid / KC + (kThreads / KC) * r;
       }
       const int64_t gi = i0 + li, gk = k0 + lk;
-      double v = (gi < g.M && gk < g.K) ? g.A[gi * g.sai + gk * g.sak] : 0.0;
-      if (g.alpha != 1.0) v = v * g.alpha;
+      F v = (gi < g.M && gk < g.K) ? g.A[gi * g.sai + gk * g.sak] : F(0);
+      if (g.alpha != F(1)) v = v * g.alpha;
       As[lk][li] = v;
     }
 #pragma unroll
@@ -169,12 +180,12 @@ RB_UNROLL_CAP
         lk = tid / BN + (kThreads / BN) * r;
       }
       const int64_t gj = j0 + lj, gk = k0 + lk;
-      Bs[lk][lj] = (gj < g.N && gk < g.K) ? g.B[gk * g.sbk + gj * g.sbj] : 0.0;
+      Bs[lk][lj] = (gj < g.N && gk < g.K) ? g.B[gk * g.sbk + gj * g.sbj] : F(0);
     }
     __syncthreads();
 
     // ---- index guard for this chunk
-    double ra = 0.0, cb = 0.0;
+    F ra = F(0), cb = F(0);
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
       const int64_t gi = i0 + tr * TM + i;
@@ -185,28 +196,28 @@ RB_UNROLL_CAP
       const int64_t gj = j0 + tc * TN + j;
       if (gj < g.N) cb = fmax(cb, g.colmax[ch * g.N + gj]);
     }
-    const double bound = (aalpha * ra) * cb;
+    const F bound = (aalpha * ra) * cb;
     if (!(bound < lmin)) {
       // exact max |fl(a*b)| of each output over the chunk, then raise; one
       // output at a time keeps this rare path's registers few
-      lmin = __builtin_inf();
+      lmin = (F)__builtin_inf();
       anytop = false;
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          double mx = 0.0;
+          F mx = F(0);
 #pragma unroll 1
           for (int k = 0; k < KC; ++k) mx = fmax(mx, fabs(As[k][tr * TM + i] * Bs[k][tc * TN + j]));
           const int cur = dmindex(p[i][j][0]);
           const int nI = dindex(mx);
           if (!nan_or_inf(p[i][j][0]) && nI < cur) raise_ic(fold, cur - nI, nI, p[i][j], c[i][j]);
           const bool live = (i0 + tr * TM + i < g.M) && (j0 + tc * TN + j < g.N);
-          if (live && !nan_or_inf(p[i][j][0])) lmin = fmin(lmin, admit_limit(dmindex(p[i][j][0])));
+          if (live && !nan_or_inf(p[i][j][0])) lmin = fmin(lmin, admit_limit<F>(dmindex(p[i][j][0])));
           anytop |= dmindex0(p[i][j][0]);
         }
     }
-    if (dep + KC > kEndurance) {
+    if (dep + KC > Fmt<F>::endurance) {
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -220,7 +231,7 @@ RB_UNROLL_CAP
       // sets instead of being copied back at the end of every trip
 #pragma unroll 2
       for (int k = 0; k < KC; ++k) {
-        double a[TM], b[TN];
+        F a[TM], b[TN];
 #pragma unroll
         for (int i = 0; i < TM; ++i) a[i] = As[k][tr * TM + i];
 #pragma unroll
@@ -233,7 +244,7 @@ RB_UNROLL_CAP
     } else {
 #pragma unroll 1
       for (int k = 0; k < KC; ++k) {
-        double a[TM], b[TN];
+        F a[TM], b[TN];
 #pragma unroll
         for (int i = 0; i < TM; ++i) a[i] = As[k][tr * TM + i];
 #pragma unroll
@@ -256,28 +267,28 @@ RB_UNROLL_CAP
       const int64_t gi = i0 + tr * TM + i, gj = j0 + tc * TN + j;
       if (gi >= g.M || gj >= g.N) continue;
       renorm_ic(fold, p[i][j], c[i][j]);
-      double ps[CAP], cs[CAP];
+      F ps[CAP], cs[CAP];
 RB_UNROLL_CAP
       for (int f = 0; f < CAP; ++f) {
         ps[f] = p[i][j][f];
-        cs[f] = (double)c[i][j][f];
+        cs[f] = (F)c[i][j][f];
       }
       if (nan_or_inf(ps[0])) {
         bool nan = false, pinf = false, ninf = false;
         for (int64_t k = 0; k < g.K; ++k) {
-          double av = g.A[gi * g.sai + k * g.sak];
-          if (g.alpha != 1.0) av = av * g.alpha;
-          const double t = av * g.B[k * g.sbk + gj * g.sbj];
+          F av = g.A[gi * g.sai + k * g.sak];
+          if (g.alpha != F(1)) av = av * g.alpha;
+          const F t = av * g.B[k * g.sbk + gj * g.sbj];
           nan |= (t != t);
-          pinf |= (t == __builtin_inf());
-          ninf |= (t == -__builtin_inf());
+          pinf |= (t == (F)__builtin_inf());
+          ninf |= (t == -(F)__builtin_inf());
         }
-        ps[0] = (nan || (pinf && ninf) || !(pinf || ninf)) ? __builtin_nan("")
-                                                            : (pinf ? __builtin_inf() : -__builtin_inf());
+        ps[0] = (nan || (pinf && ninf) || !(pinf || ninf)) ? (F)__builtin_nan("")
+                                                            : (pinf ? (F)__builtin_inf() : -(F)__builtin_inf());
       }
       if (g.Cb) {
-        double* cbp = g.Cb + (gi * g.cbi + gj * g.cbj) * 2 * fold;
-        double py[CAP], cy[CAP];
+        F* cbp = g.Cb + (gi * g.cbi + gj * g.cbj) * 2 * fold;
+        F py[CAP], cy[CAP];
 RB_UNROLL_CAP
         for (int f = 0; f < CAP; ++f)
           if (f < fold) { py[f] = cbp[f]; cy[f] = cbp[fold + f]; }
@@ -288,21 +299,21 @@ RB_UNROLL_CAP
         for (int f = 0; f < CAP; ++f)
           if (f < fold) { cbp[f] = py[f]; cbp[fold + f] = cy[f]; }
       } else {
-        double* cp = g.C + gi * g.sci + gj * g.scj;
-        double py[CAP], cy[CAP];
-        beta_init(fold, g.beta, g.beta == 0.0 ? 0.0 : *cp, py, cy);
+        F* cp = g.C + gi * g.sci + gj * g.scj;
+        F py[CAP], cy[CAP];
+        beta_init(fold, g.beta, g.beta == F(0) ? F(0) : *cp, py, cy);
         add(fold, py, cy, ps, cs);
         renorm(fold, py, cy);
-        *cp = to_double(fold, py, cy);
+        *cp = convert(fold, py, cy);
       }
     }
   }
 }
 
-template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK>
-__global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc g) {
-  __shared__ __attribute__((aligned(16))) double As[KC][16 * TM + kPad];
-  __shared__ __attribute__((aligned(16))) double Bs[KC][16 * TN + kPad];
+template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK, class F>
+__global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDescT<F> g) {
+  __shared__ __attribute__((aligned(16))) F As[KC][16 * TM + pad<F>()];
+  __shared__ __attribute__((aligned(16))) F Bs[KC][16 * TN + pad<F>()];
   gemm_tile<CAP, FIXED, TM, TN, AI, BK>(g, (int64_t)blockIdx.y * 16 * TM, (int64_t)blockIdx.x * 16 * TN, As, Bs);
 }
 
@@ -313,8 +324,8 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc g) {
 template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK>
 __global__ __launch_bounds__(kThreads) void gemm_batch_kernel(const GemmDesc* descs, const GemmTile* tiles,
                                                               int64_t ntiles) {
-  __shared__ __attribute__((aligned(16))) double As[KC][16 * TM + kPad];
-  __shared__ __attribute__((aligned(16))) double Bs[KC][16 * TN + kPad];
+  __shared__ __attribute__((aligned(16))) double As[KC][16 * TM + pad<double>()];
+  __shared__ __attribute__((aligned(16))) double Bs[KC][16 * TN + pad<double>()];
   if ((int64_t)blockIdx.x >= ntiles) return;
   const GemmTile tl = tiles[blockIdx.x];
   const GemmDesc g = descs[tl.gemm];
@@ -340,13 +351,13 @@ __global__ __launch_bounds__(256) void chunk_max_batch_kernel(const GemmDesc* de
   }
 }
 
-template <int CAP, bool FIXED, int TM, int TN>
-bool launch_tm(bool ai, bool bk, const GemmDesc& g, hipStream_t s) {
+template <int CAP, bool FIXED, int TM, int TN, class F>
+bool launch_tm(bool ai, bool bk, const GemmDescT<F>& g, hipStream_t s) {
   dim3 grid((unsigned)((g.N + 16 * TN - 1) / (16 * TN)), (unsigned)((g.M + 16 * TM - 1) / (16 * TM)));
-  if (ai && bk) gemm_kernel<CAP, FIXED, TM, TN, true, true><<<grid, kThreads, 0, s>>>(g);
-  else if (ai) gemm_kernel<CAP, FIXED, TM, TN, true, false><<<grid, kThreads, 0, s>>>(g);
-  else if (bk) gemm_kernel<CAP, FIXED, TM, TN, false, true><<<grid, kThreads, 0, s>>>(g);
-  else gemm_kernel<CAP, FIXED, TM, TN, false, false><<<grid, kThreads, 0, s>>>(g);
+  if (ai && bk) gemm_kernel<CAP, FIXED, TM, TN, true, true, F><<<grid, kThreads, 0, s>>>(g);
+  else if (ai) gemm_kernel<CAP, FIXED, TM, TN, true, false, F><<<grid, kThreads, 0, s>>>(g);
+  else if (bk) gemm_kernel<CAP, FIXED, TM, TN, false, true, F><<<grid, kThreads, 0, s>>>(g);
+  else gemm_kernel<CAP, FIXED, TM, TN, false, false, F><<<grid, kThreads, 0, s>>>(g);
   return RB_HIP(hipGetLastError());
 }
 
@@ -364,8 +375,8 @@ int g_tiling = 1;  // 0: 4x4 outputs per lane, 1: 4x2, 2: 2x2
 // Tiling actually used for a fold (folds without a dedicated instance use 2x2).
 int effective_tiling(int tiling, int fold) { return (fold >= 2 && fold <= 4) ? tiling : 2; }
 
-template <int CAP, bool FIXED>
-bool launch_fold(int tiling, bool ai, bool bk, const GemmDesc& g, hipStream_t s) {
+template <int CAP, bool FIXED, class F>
+bool launch_fold(int tiling, bool ai, bool bk, const GemmDescT<F>& g, hipStream_t s) {
   if constexpr (!FIXED) {
     return launch_tm<CAP, FIXED, 2, 2>(ai, bk, g, s);  // folds 5..8: fewest registers
   } else {
@@ -391,15 +402,16 @@ bool launch_batch_fold(int tiling, bool ai, bool bk, const GemmDesc* d, const Ge
   }
 }
 
-bool gemm_common(const GemmDesc& g0, hipStream_t s) {
+template <class F>
+bool gemm_common(const GemmDescT<F>& g0, hipStream_t s) {
   const int fold = g0.fold;
-  if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
+  if (fold < 2 || fold > Fmt<F>::max_fold) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, %d]", fold, Fmt<F>::max_fold);
     return false;
   }
-  GemmDesc g = g0;
+  GemmDescT<F> g = g0;
   const int64_t chunks = (g.K + KC - 1) / KC;
-  double* mx = (double*)workspace(s, kSlotGemmMax, sizeof(double) * (size_t)chunks * (g.M + g.N));
+  F* mx = (F*)workspace(s, kSlotGemmMax, sizeof(F) * (size_t)chunks * (g.M + g.N));
   if (!mx) return false;
   g.rowmax = mx;
   g.colmax = mx + chunks * g.M;
@@ -414,19 +426,21 @@ bool gemm_common(const GemmDesc& g0, hipStream_t s) {
     case 4: return launch_fold<4, true>(g_tiling, ai, bk, g, s);
     default:
       if (fold <= 8) return launch_fold<8, false>(g_tiling, ai, bk, g, s);
-      return launch_fold<kMaxFold, false>(g_tiling, ai, bk, g, s);
+      return launch_fold<Fmt<F>::max_fold, false>(g_tiling, ai, bk, g, s);
   }
 }
 
 }  // namespace
 
-GemmDesc make_gemm_desc(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
-                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
-                        double* Cb, int64_t ldcb) {
+namespace {
+template <class F>
+GemmDescT<F> make_desc(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, F alpha,
+                       const F* A, int64_t lda, const F* B, int64_t ldb, F beta, F* C, int64_t ldc, F* Cb,
+                       int64_t ldcb) {
   const bool rm = (order == 'r' || order == 'R');
   const bool na = (transa == 'n' || transa == 'N');
   const bool nb = (transb == 'n' || transb == 'N');
-  GemmDesc g;
+  GemmDescT<F> g;
   g.M = M;
   g.N = N;
   g.K = K;
@@ -443,6 +457,34 @@ GemmDesc make_gemm_desc(int fold, char order, char transa, char transb, int64_t 
   g.rowmax = g.colmax = nullptr;
   g.fold = fold;
   return g;
+}
+
+// rdgemm.c:41-105 / rsgemm.c: C = conv(binned(beta*C) (+) sum) in place
+template <class F>
+bool rgemm_impl(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, F alpha, const F* A,
+                int64_t lda, const F* B, int64_t ldb, F beta, F* C, int64_t ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return true;  // rdgemm.c:50-52
+  if (K <= 0 || alpha == F(0)) {
+    // no products: C still goes through binned(beta*C) and back
+    const bool rm = (order == 'r' || order == 'R');
+    const int64_t outer = rm ? M : N, inner = rm ? N : M;
+    F* tmp = (F*)workspace(s, kSlotBinnedOut, sizeof(F) * (size_t)inner * 2 * fold);
+    if (!tmp) return false;
+    for (int64_t o = 0; o < outer; ++o) {
+      if (!conv_to_binned(fold, inner, C + o * ldc, 1, beta, tmp, s)) return false;
+      if (!conv_from_binned(fold, inner, tmp, C + o * ldc, 1, s)) return false;
+    }
+    return true;
+  }
+  return gemm_common(make_desc(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                               (F*)nullptr, 0), s);
+}
+}  // namespace
+
+GemmDesc make_gemm_desc(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                        double* Cb, int64_t ldcb) {
+  return make_desc<double>(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, Cb, ldcb);
 }
 
 void gemm_tile_shape(int tiling, int fold, int* bm, int* bn) {
@@ -493,28 +535,27 @@ int gemm_get_tiling() { return g_tiling; }
 bool dbdgemm(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
              const double* A, int64_t lda, const double* B, int64_t ldb, double* Cb, int64_t ldc, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0 || alpha == 0.0) return true;  // dbdgemm.ccog:77-80
-  return gemm_common(
-      make_gemm_desc(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, 0.0, nullptr, 0, Cb, ldc), s);
+  return gemm_common(make_desc<double>(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, 0.0, nullptr, 0,
+                                       Cb, ldc), s);
+}
+
+bool sbsgemm(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, float alpha,
+             const float* A, int64_t lda, const float* B, int64_t ldb, float* Cb, int64_t ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0 || alpha == 0.0f) return true;  // sbsgemm.ccog:78-80
+  return gemm_common(make_desc<float>(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, 0.0f, nullptr, 0,
+                                      Cb, ldc), s);
 }
 
 bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
                    hipStream_t s) {
-  if (M <= 0 || N <= 0) return true;  // rdgemm.c:50-52
-  if (K <= 0 || alpha == 0.0) {
-    // no products: C still goes through binned(beta*C) and back
-    const bool rm = (order == 'r' || order == 'R');
-    const int64_t outer = rm ? M : N, inner = rm ? N : M;
-    double* tmp = (double*)workspace(s, kSlotBinnedOut, sizeof(double) * (size_t)inner * 2 * fold);
-    if (!tmp) return false;
-    for (int64_t o = 0; o < outer; ++o) {
-      if (!conv_to_binned(fold, inner, C + o * ldc, 1, beta, tmp, s)) return false;
-      if (!conv_to_double(fold, inner, tmp, C + o * ldc, 1, s)) return false;
-    }
-    return true;
-  }
-  return gemm_common(
-      make_gemm_desc(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0), s);
+  return rgemm_impl<double>(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+bool rsgemm_device(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, float alpha,
+                   const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
+                   hipStream_t s) {
+  return rgemm_impl<float>(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 }  // namespace rb

@@ -1,10 +1,13 @@
-// host_binned.hip -- the binned_* primitives of include/binned.h.
+// host_binned.hip -- the binned_* primitives of include/binned.h (double_binned
+// and float_binned).
 //
 // These act on one binned value (a few dozen bytes) in host memory and run on
 // the host, as the reference's do; they share their arithmetic with the
 // kernels through binned_core.hpp (compiled for both sides with
 // -ffp-contract=off). binned_dbdbaddv on device arrays runs on the GPU.
+#include <float.h>
 #include <math.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -18,33 +21,41 @@ using namespace rb;
 namespace {
 
 constexpr int CAP = kMaxFold;
+constexpr int SCAP = Fmt<float>::max_fold;
 
-struct Local {
-  double p[CAP];
-  double c[CAP];
+template <class F, int N>
+struct LocalT {
+  F p[N];
+  F c[N];
 };
+using Local = LocalT<double, CAP>;
+using LocalS = LocalT<float, SCAP>;
 
-inline void gather(int fold, const double* pri, int incp, const double* car, int incc, Local& v) {
+template <class F, int N>
+inline void gather(int fold, const F* pri, int incp, const F* car, int incc, LocalT<F, N>& v) {
   for (int i = 0; i < fold; ++i) {
     v.p[i] = pri[(long)i * incp];
     v.c[i] = car[(long)i * incc];
   }
-  for (int i = fold; i < CAP; ++i) v.p[i] = v.c[i] = 0.0;
+  for (int i = fold; i < N; ++i) v.p[i] = v.c[i] = F(0);
 }
-inline void scatter(int fold, const Local& v, double* pri, int incp, double* car, int incc) {
+template <class F, int N>
+inline void scatter(int fold, const LocalT<F, N>& v, F* pri, int incp, F* car, int incc) {
   for (int i = 0; i < fold; ++i) {
     pri[(long)i * incp] = v.p[i];
     car[(long)i * incc] = v.c[i];
   }
 }
 
+template <class F>
 struct BinTable {
-  double v[kMaxIndex + kMaxFold];
+  F v[Fmt<F>::max_index + Fmt<F>::max_fold];
   BinTable() {
-    for (int i = 0; i < kMaxIndex + kMaxFold; ++i) v[i] = bin(i);
+    for (int i = 0; i < Fmt<F>::max_index + Fmt<F>::max_fold; ++i) v[i] = bin<F>(i);
   }
 };
-const BinTable g_bins;  // built at load time: no lazy initialisation race
+const BinTable<double> g_bins;  // built at load time: no lazy initialisation race
+const BinTable<float> g_sbins;
 
 // the reference's UFP returns float (src/common/common.h:29-34), so
 // binned_ufp narrows through float
@@ -238,6 +249,190 @@ double binned_dmdmaddsq(const int fold, const double scaleX, const double* priX,
 double binned_dbdbaddsq(const int fold, const double scaleX, const double_binned* X, const double scaleY,
                         double_binned* Y) {
   return binned_dmdmaddsq(fold, scaleX, X, 1, X + fold, 1, scaleY, Y, 1, Y + fold, 1);
+}
+
+
+// ------------------------------------------------------------ float_binned --
+// reference src/binned/sbsbze.c, sballoc.c, sbnum.c
+size_t binned_sbsbze(const int fold) { return 2 * (size_t)fold * sizeof(float); }
+float_binned* binned_sballoc(const int fold) { return (float_binned*)malloc(binned_sbsbze(fold)); }
+int binned_sbnum(const int fold) { return 2 * fold; }
+
+// reference src/binned/sbbound.c
+float binned_sbbound(const int fold, const int N, const float X, const float S) {
+  const double a = fabs((double)X) > ldexp(0.5, FLT_MIN_EXP - 1) ? fabs((double)X) : ldexp(0.5, FLT_MIN_EXP - 1);
+  return (float)(a * ldexp(0.5, (1 - fold) * SBWIDTH + 1) * N +
+                 ((7.0 * FLT_EPSILON) / (1.0 - 6.0 * sqrt((double)FLT_EPSILON) - 7.0 * FLT_EPSILON)) *
+                     fabs((double)S));
+}
+
+// reference src/binned/smbins.c, sindex.c, smindex.c, smindex0.c
+const float* binned_smbins(const int X) { return g_sbins.v + X; }
+int binned_sindex(const float X) { return dindex(X); }
+int binned_smindex(const float* priX) { return dmindex(priX[0]); }
+int binned_smindex0(const float* priX) { return dmindex0(priX[0]) ? 1 : 0; }
+// reference src/binned/smdenorm.c
+int binned_smdenorm(const int fold, const float* priX) {
+  return biased_exp(priX[0]) - fold * SBWIDTH < FLT_MIN_EXP + FLT_MANT_DIG + (FLT_MAX_EXP - 2) + SBWIDTH;
+}
+// reference src/binned/ufpf.c (UFPF: the exponent field kept)
+float binned_ufpf(const float X) { return fp_from_bits<float>(bits(X) & (0xffu << 23)); }
+
+// reference src/binned/smprint.c
+void binned_smprint(const int fold, const float* priX, const int incpriX, const float* carX, const int inccarX) {
+  for (int i = 0; i < fold; ++i) {
+    const float p = priX[(long)i * incpriX], c = carX[(long)i * inccarX];
+    const float M = binned_ufpf(p);
+    printf("(2^%d: %g #%g =%g)\n", (int)log2f(M) + 1, p - 1.5 * M, c, ((c - 6) * 0.25 * M + p));
+  }
+}
+void binned_sbprint(const int fold, const float_binned* X) { binned_smprint(fold, X, 1, X + fold, 1); }
+
+// reference src/binned/smsmset.c, sbsbset.c, smsetzero.c, sbsetzero.c
+void binned_smsmset(const int fold, const float* priX, const int incpriX, const float* carX, const int inccarX,
+                    float* priY, const int incpriY, float* carY, const int inccarY) {
+  for (int i = 0; i < fold; ++i) {
+    priY[(long)i * incpriY] = priX[(long)i * incpriX];
+    carY[(long)i * inccarY] = carX[(long)i * inccarX];
+  }
+}
+void binned_sbsbset(const int fold, const float_binned* X, float_binned* Y) {
+  binned_smsmset(fold, X, 1, X + fold, 1, Y, 1, Y + fold, 1);
+}
+void binned_smsetzero(const int fold, float* priX, const int incpriX, float* carX, const int inccarX) {
+  for (int i = 0; i < fold; ++i) {
+    priX[(long)i * incpriX] = 0.0f;
+    carX[(long)i * inccarX] = 0.0f;
+  }
+}
+void binned_sbsetzero(const int fold, float_binned* X) { memset(X, 0, binned_sbsbze(fold)); }
+
+// reference src/binned/smsmadd.c, sbsbadd.c
+void binned_smsmadd(const int fold, const float* priX, const int incpriX, const float* carX, const int inccarX,
+                    float* priY, const int incpriY, float* carY, const int inccarY) {
+  LocalS x, y;
+  gather(fold, priX, incpriX, carX, inccarX, x);
+  gather(fold, priY, incpriY, carY, inccarY, y);
+  add(fold, y.p, y.c, x.p, x.c);
+  scatter(fold, y, priY, incpriY, carY, inccarY);
+}
+void binned_sbsbadd(const int fold, const float_binned* X, float_binned* Y) {
+  binned_smsmadd(fold, X, 1, X + fold, 1, Y, 1, Y + fold, 1);
+}
+
+// reference src/binned/sbsbaddv.c; device arrays run on the GPU
+void binned_sbsbaddv(const int fold, const int N, const float_binned* X, const int incX, float_binned* Y,
+                     const int incY) {
+  clear_error();
+  if (N <= 0) return;
+  const bool dx = is_device_pointer(X), dy = is_device_pointer(Y);
+  if (dx && dy) {
+    hipStream_t s = current_stream();
+    if (sbsbaddv(fold, N, X, incX, Y, incY, s)) RB_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  if (dx != dy) {
+    set_error(kBadArgument, "binned_sbsbaddv: X and Y must both be host or both be device arrays");
+    return;
+  }
+  for (long i = 0; i < N; ++i)
+    binned_sbsbadd(fold, X + i * incX * 2 * fold, Y + i * incY * 2 * fold);
+}
+
+// reference src/binned/smsupdate.c, sbsupdate.c
+void binned_smsupdate(const int fold, const float X, float* priY, const int incpriY, float* carY, const int inccarY) {
+  LocalS y;
+  gather(fold, priY, incpriY, carY, inccarY, y);
+  update(fold, X, y.p, y.c);
+  scatter(fold, y, priY, incpriY, carY, inccarY);
+}
+void binned_sbsupdate(const int fold, const float X, float_binned* Y) {
+  binned_smsupdate(fold, X, Y, 1, Y + fold, 1);
+}
+
+// reference src/binned/smsdeposit.c, sbsdeposit.c
+void binned_smsdeposit(const int fold, const float X, float* priY, const int incpriY) {
+  if (nan_or_inf(X) || nan_or_inf(priY[0])) {
+    priY[0] += X;
+    return;
+  }
+  LocalS y;
+  gather(fold, priY, incpriY, priY, incpriY, y);
+  deposit(fold, dmindex0(y.p[0]), X, y.p);
+  for (int i = 0; i < fold; ++i) priY[(long)i * incpriY] = y.p[i];
+}
+void binned_sbsdeposit(const int fold, const float X, float_binned* Y) { binned_smsdeposit(fold, X, Y, 1); }
+
+// reference src/binned/smrenorm.c, sbrenorm.c
+void binned_smrenorm(const int fold, float* priX, const int incpriX, float* carX, const int inccarX) {
+  LocalS x;
+  gather(fold, priX, incpriX, carX, inccarX, x);
+  renorm(fold, x.p, x.c);
+  scatter(fold, x, priX, incpriX, carX, inccarX);
+}
+void binned_sbrenorm(const int fold, float_binned* X) { binned_smrenorm(fold, X, 1, X + fold, 1); }
+
+// reference src/binned/smsadd.c, sbsadd.c
+void binned_smsadd(const int fold, const float X, float* priY, const int incpriY, float* carY, const int inccarY) {
+  binned_smsupdate(fold, X, priY, incpriY, carY, inccarY);
+  binned_smsdeposit(fold, X, priY, incpriY);
+  binned_smrenorm(fold, priY, incpriY, carY, inccarY);
+}
+void binned_sbsadd(const int fold, const float X, float_binned* Y) { binned_smsadd(fold, X, Y, 1, Y + fold, 1); }
+
+// reference src/binned/smsconv.c, sbsconv.c
+void binned_smsconv(const int fold, const float X, float* priY, const int incpriY, float* carY, const int inccarY) {
+  binned_smsetzero(fold, priY, incpriY, carY, inccarY);
+  binned_smsadd(fold, X, priY, incpriY, carY, inccarY);
+}
+void binned_sbsconv(const int fold, const float X, float_binned* Y) { binned_smsconv(fold, X, Y, 1, Y + fold, 1); }
+
+// reference src/binned/ssmconv.c, ssbconv.c
+float binned_ssmconv(const int fold, const float* priX, const int incpriX, const float* carX, const int inccarX) {
+  LocalS x;
+  gather(fold, priX, incpriX, carX, inccarX, x);
+  return convert(fold, x.p, x.c);
+}
+float binned_ssbconv(const int fold, const float_binned* X) { return binned_ssmconv(fold, X, 1, X + fold, 1); }
+
+// reference src/binned/smnegate.c, sbnegate.c
+void binned_smnegate(const int fold, float* priX, const int incpriX, float* carX, const int inccarX) {
+  if (priX[0] == 0.0f) return;
+  const int I = dmindex(priX[0]);
+  for (int i = 0; i < fold; ++i) {
+    const float b = bin<float>(I + i);
+    priX[(long)i * incpriX] = b - (priX[(long)i * incpriX] - b);
+    carX[(long)i * inccarX] = -carX[(long)i * inccarX];
+  }
+}
+void binned_sbnegate(const int fold, float_binned* X) { binned_smnegate(fold, X, 1, X + fold, 1); }
+
+// reference src/binned/sscale.c
+float binned_sscale(const float X) { return dscale(X); }
+
+// reference src/binned/smsrescale.c
+void binned_smsrescale(const int fold, const float X, const float scaleY, float* priY, const int incpriY, float* carY,
+                       const int inccarY) {
+  LocalS y;
+  gather(fold, priY, incpriY, carY, inccarY, y);
+  rescale(fold, X, scaleY, y.p, y.c);
+  scatter(fold, y, priY, incpriY, carY, inccarY);
+}
+
+// reference src/binned/smsmaddsq.c, sbsbaddsq.c
+float binned_smsmaddsq(const int fold, const float scaleX, const float* priX, const int incpriX, const float* carX,
+                       const int inccarX, const float scaleY, float* priY, const int incpriY, float* carY,
+                       const int inccarY) {
+  LocalS x, y;
+  gather(fold, priX, incpriX, carX, inccarX, x);
+  gather(fold, priY, incpriY, carY, inccarY, y);
+  const float sc = add_scaled(fold, scaleX, x.p, x.c, scaleY, y.p, y.c);
+  scatter(fold, y, priY, incpriY, carY, inccarY);
+  return sc;
+}
+float binned_sbsbaddsq(const int fold, const float scaleX, const float_binned* X, const float scaleY,
+                       float_binned* Y) {
+  return binned_smsmaddsq(fold, scaleX, X, 1, X + fold, 1, scaleY, Y, 1, Y + fold, 1);
 }
 
 }  // extern "C"

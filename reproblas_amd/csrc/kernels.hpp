@@ -24,6 +24,10 @@ enum Term : int { kSum = 0, kASum = 1, kDot = 2, kSSq = 3 };
 bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
            const double* state_in, double scale_in, const double* scale_in_dev, double* state_out,
            double* scale_out, double* result, hipStream_t s);
+// The same for float operands and float_binned values (folds 2..21).
+bool blas1_s(int fold, Term term, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy,
+             const float* state_in, float scale_in, const float* scale_in_dev, float* state_out, float* scale_out,
+             float* result, hipStream_t s);
 
 // Ragged batch of BLAS1 reductions over CSR segments of x (and y for dot),
 // unit stride. chunks[c] = [begin, end) element range (at most a few
@@ -37,11 +41,15 @@ bool segments(int fold, Term term, const double* x, const double* y, const SegCh
               double* partials, const int64_t* first_chunk, int64_t nseg, double* results, int grid, hipStream_t s);
 
 // Elementwise binned helpers on device arrays of binned values.
-// dbdbaddv: Y[i] += X[i] for i < n (stride incX/incY in binned elements).
+// dbdbaddv / sbsbaddv: Y[i] += X[i] for i < n (stride incX/incY in binned elements).
 bool dbdbaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int64_t incY, hipStream_t s);
-// conv_to_binned: Y[i] = dbdconv(beta == 1 ? y[i] : y[i]*beta) or zero when beta == 0.
+bool sbsbaddv(int fold, int64_t n, const float* X, int64_t incX, float* Y, int64_t incY, hipStream_t s);
+// conv_to_binned: Y[i] = dbdconv/sbsconv(beta == 1 ? y[i] : y[i]*beta) or zero when beta == 0.
 bool conv_to_binned(int fold, int64_t n, const double* y, int64_t incy, double beta, double* Yb, hipStream_t s);
-// conv_to_double: y[i] = ddbconv(Yb[i]).
+bool conv_to_binned(int fold, int64_t n, const float* y, int64_t incy, float beta, float* Yb, hipStream_t s);
+// conv_from_binned: y[i] = ddbconv/ssbconv(Yb[i]); conv_to_double is the double one.
+bool conv_from_binned(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s);
+bool conv_from_binned(int fold, int64_t n, const float* Yb, float* y, int64_t incy, hipStream_t s);
 bool conv_to_double(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s);
 
 // Binned gemv on device operands (reference binnedBLAS_dbdgemv semantics,
@@ -54,6 +62,17 @@ bool dbdgemv(int fold, char order, char trans, int64_t M, int64_t N, double alph
 // leading dimension ldc (in binned elements).
 bool dbdgemm(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
              const double* A, int64_t lda, const double* B, int64_t ldb, double* Cb, int64_t ldc, hipStream_t s);
+
+// Float counterparts (binnedBLAS_sbsgemv / sbsgemm, reproBLAS_rsgemv / rsgemm).
+bool sbsgemv(int fold, char order, char trans, int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
+             const float* x, int64_t incx, float* Yb, int64_t incYb, hipStream_t s);
+bool sbsgemm(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, float alpha,
+             const float* A, int64_t lda, const float* B, int64_t ldb, float* Cb, int64_t ldc, hipStream_t s);
+bool rsgemv_device(int fold, char order, char trans, int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
+                   const float* x, int64_t incx, float beta, float* y, int64_t incy, hipStream_t s);
+bool rsgemm_device(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, float alpha,
+                   const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
+                   hipStream_t s);
 
 // In-place y = conv(binned(beta*y) (+) sum_j fl(op(A)_ij * fl(alpha*x_j)))
 // (reproBLAS_rdgemv) on device operands.
@@ -71,22 +90,24 @@ bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, in
 //   C(i,j) = C[i*sci + j*scj] (double output, beta applied) or, when Cb is
 //   set, the binned output Cb[(i*cbi + j*cbj)*2*fold..] accumulated into.
 // rowmax/colmax: scratch of ceil(K/32)*M and ceil(K/32)*N doubles.
-struct GemmDesc {
+template <class F>
+struct GemmDescT {
   int64_t M, N, K;
-  const double* A;
+  const F* A;
   int64_t sai, sak;
-  const double* B;
+  const F* B;
   int64_t sbk, sbj;
-  double alpha;
-  double* C;
+  F alpha;
+  F* C;
   int64_t sci, scj;
-  double beta;
-  double* Cb;
+  F beta;
+  F* Cb;
   int64_t cbi, cbj;
-  const double* rowmax;
-  const double* colmax;
+  const F* rowmax;
+  const F* colmax;
   int fold;
 };
+using GemmDesc = GemmDescT<double>;
 struct GemmTile {
   int32_t gemm, ti, tj, pad;
 };

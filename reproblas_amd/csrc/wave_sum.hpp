@@ -1,6 +1,6 @@
-// wave_sum.hpp -- one wave's binned reduction of a strided vector range,
-// shared by the BLAS1 kernels, the ragged segment kernels and the row-dot
-// gemv kernel.
+// wave_sum.hpp -- one wave's binned reduction of a strided vector range of
+// doubles or floats, shared by the BLAS1 kernels, the ragged segment kernels
+// and the row-dot gemv kernel.
 //
 // Reference path: the per-block loops of binnedBLAS_dmdsum
 // (src/binnedBLAS/dmdsum.ccog:57-89), dmdasum (dmdasum.ccog:57-85), dmddot
@@ -63,17 +63,25 @@ __device__ __forceinline__ double lane_partner(double v, int st) {
   const uint32_t hi = (uint32_t)lane_partner32((int)(uint32_t)(u >> 32), st);
   return from_bits(((uint64_t)hi << 32) | lo);
 }
+__device__ __forceinline__ float lane_partner(float v, int st) {
+  return fp_from_bits<float>((uint32_t)lane_partner32((int)bits(v), st));
+}
 
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__device__ __forceinline__ double uniform_double(double v) {
+__device__ __forceinline__ double uniform_value(double v) {
   const uint64_t u = bits(v);
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
   return from_bits(((uint64_t)hi << 32) | lo);
 }
+__device__ __forceinline__ float uniform_value(float v) {
+  return fp_from_bits<float>((uint32_t)__builtin_amdgcn_readfirstlane((int)bits(v)));
+}
+__device__ __forceinline__ double uniform_double(double v) { return uniform_value(v); }
 
-__device__ __forceinline__ double wave_max(double v) {
+template <class F>
+__device__ __forceinline__ F wave_max(F v) {
 #pragma unroll
   for (int st = 0; st < 6; ++st) v = fmax(v, lane_partner(v, st));
   return v;
@@ -81,29 +89,31 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // The term as the reference's infinity path adds it (dmdsum.ccog:68-72,
 // dmdasum and dmdssq use fabs, dmddot the product).
-template <int T>
-__device__ __forceinline__ double plain_term(double x, double y) {
-  if (T == kSum) return x;
-  if (T == kDot) return x * y;
+template <int K, class F>
+__device__ __forceinline__ F plain_term(F x, F y) {
+  if (K == kSum) return x;
+  if (K == kDot) return x * y;
   return fabs(x);
 }
 
 typedef double d2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
-// A strided vector range walked in wave tiles of 64*U terms.
+// A strided vector range of element type F walked in wave tiles of 64*U terms.
+template <class F>
 struct Span {
-  const double* x;
+  const F* x;
   int64_t incx;
-  const double* y;  // second operand (dot) or null
+  const F* y;  // second operand (dot) or null
   int64_t incy;
   int64_t n;
   int64_t tiles;       // tiles covering n
   int64_t full_tiles;  // tiles with no out-of-range element
 };
 
-template <int U>
-__device__ __forceinline__ Span make_span(const double* x, int64_t incx, const double* y, int64_t incy, int64_t n) {
-  Span sp;
+template <int U, class F>
+__device__ __forceinline__ Span<F> make_span(const F* x, int64_t incx, const F* y, int64_t incy, int64_t n) {
+  Span<F> sp;
   sp.x = x;
   sp.incx = incx;
   sp.y = y;
@@ -114,32 +124,42 @@ __device__ __forceinline__ Span make_span(const double* x, int64_t incx, const d
   return sp;
 }
 
+// U consecutive 16-byte vectors of a lane (64 lanes interleaved per vector
+// row), streamed with non-temporal loads: every byte is read once.
+template <int U, class F>
+__device__ __forceinline__ void load_rows16(const F* base, int lane, F (&v)[U]) {
+  if constexpr (sizeof(F) == 8) {
+    const d2* p = reinterpret_cast<const d2*>(base);
+#pragma unroll
+    for (int k = 0; k < U / 2; ++k) {
+      const d2 d = __builtin_nontemporal_load(p + k * 64 + lane);
+      v[2 * k] = d.x;
+      v[2 * k + 1] = d.y;
+    }
+  } else {
+    const f4* p = reinterpret_cast<const f4*>(base);
+#pragma unroll
+    for (int k = 0; k < U / 4; ++k) {
+      const f4 d = __builtin_nontemporal_load(p + k * 64 + lane);
+      v[4 * k] = d.x;
+      v[4 * k + 1] = d.y;
+      v[4 * k + 2] = d.z;
+      v[4 * k + 3] = d.w;
+    }
+  }
+}
+
 // Loads the U raw terms of tile t for this lane. MODE 0: unit stride with
-// 16-byte vector loads (operands 16-byte aligned); MODE 1: unit stride, 8-byte
-// loads; MODE 2: arbitrary stride. Out-of-range terms read as 0, which every
-// deposit leaves unchanged.
-template <int T, int U, int MODE>
-__device__ __forceinline__ void load_tile(const Span& a, int64_t t, int lane, double (&vx)[U], double (&vy)[U]) {
+// 16-byte vector loads (operands 16-byte aligned); MODE 1: unit stride,
+// element loads; MODE 2: arbitrary stride. Out-of-range terms read as 0,
+// which every deposit leaves unchanged.
+template <int K, int U, int MODE, class F>
+__device__ __forceinline__ void load_tile(const Span<F>& a, int64_t t, int lane, F (&vx)[U], F (&vy)[U]) {
   const int64_t base = t * (64 * U);
   const bool full = t < a.full_tiles;
   if (MODE == 0 && full) {
-    // streaming 16-byte loads (non-temporal: every byte is read once)
-    const d2* xv = reinterpret_cast<const d2*>(a.x + base);
-#pragma unroll
-    for (int k = 0; k < U / 2; ++k) {
-      const d2 d = __builtin_nontemporal_load(xv + k * 64 + lane);
-      vx[2 * k] = d.x;
-      vx[2 * k + 1] = d.y;
-    }
-    if (T == kDot) {
-      const d2* yv = reinterpret_cast<const d2*>(a.y + base);
-#pragma unroll
-      for (int k = 0; k < U / 2; ++k) {
-        const d2 d = __builtin_nontemporal_load(yv + k * 64 + lane);
-        vy[2 * k] = d.x;
-        vy[2 * k + 1] = d.y;
-      }
-    }
+    load_rows16<U>(a.x + base, lane, vx);
+    if (K == kDot) load_rows16<U>(a.y + base, lane, vy);
     return;
   }
 #pragma unroll
@@ -147,23 +167,23 @@ __device__ __forceinline__ void load_tile(const Span& a, int64_t t, int lane, do
     const int64_t e = base + k * 64 + lane;
     const bool in = full || e < a.n;
     const int64_t ix = (MODE == 2) ? e * a.incx : e;
-    vx[k] = in ? a.x[ix] : 0.0;
-    if (T == kDot) {
+    vx[k] = in ? a.x[ix] : F(0);
+    if (K == kDot) {
       const int64_t iy = (MODE == 2) ? e * a.incy : e;
-      vy[k] = in ? a.y[iy] : 0.0;
+      vy[k] = in ? a.y[iy] : F(0);
     }
   }
 }
 
-template <int CAP>
+template <int CAP, class F = double>
 struct State {
-  double p[CAP];
-  double c[CAP];
-  double scl;  // nrm2 only
+  F p[CAP];
+  F c[CAP];
+  F scl;  // nrm2 only
 };
 
-template <int CAP>
-__device__ __forceinline__ void store_state(double* dst, int fold, const State<CAP>& s) {
+template <int CAP, class F>
+__device__ __forceinline__ void store_state(F* dst, int fold, const State<CAP, F>& s) {
 RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
@@ -173,8 +193,8 @@ RB_UNROLL_CAP
   dst[2 * fold] = s.scl;
 }
 
-template <int CAP>
-__device__ __forceinline__ void load_state(const double* src, int fold, State<CAP>& s) {
+template <int CAP, class F>
+__device__ __forceinline__ void load_state(const F* src, int fold, State<CAP, F>& s) {
 RB_UNROLL_CAP
   for (int j = 0; j < CAP; ++j)
     if (j < fold) {
@@ -185,9 +205,9 @@ RB_UNROLL_CAP
 }
 
 // General merge of two renormalised values (with scales for nrm2).
-template <int CAP, int T>
-__device__ __forceinline__ void merge(int fold, State<CAP>& y, const State<CAP>& x) {
-  if (T == kSSq) {
+template <int CAP, int K, class F>
+__device__ __forceinline__ void merge(int fold, State<CAP, F>& y, const State<CAP, F>& x) {
+  if (K == kSSq) {
     if (nan_or_inf(x.p[0]) || nan_or_inf(y.p[0])) {
       // classes combine by addition; the scale no longer matters
       if (!nan_or_inf(y.p[0])) {
@@ -206,89 +226,90 @@ RB_UNROLL_CAP
 }
 
 // Merge the 64 lanes of a wave (general add: lanes may differ in index).
-template <int CAP, int T>
-__device__ __forceinline__ void wave_merge_general(int fold, State<CAP>& s) {
+template <int CAP, int K, class F>
+__device__ __forceinline__ void wave_merge_general(int fold, State<CAP, F>& s) {
 #pragma unroll
   for (int st = 0; st < 6; ++st) {
-    State<CAP> o;
+    State<CAP, F> o;
 RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
       o.p[j] = lane_partner(s.p[j], st);
       o.c[j] = lane_partner(s.c[j], st);
     }
     o.scl = lane_partner(s.scl, st);
-    merge<CAP, T>(fold, s, o);
+    merge<CAP, K>(fold, s, o);
   }
 }
 
 // Class of the terms of the tiles this wave walked: NaN if any term is NaN or
 // both infinities occur, else the infinity that occurs. Terms as the
 // reference's infinity path forms them.
-template <int T, int U, int MODE>
-__device__ double wave_term_class(const Span& a, int64_t first, int64_t step, int lane) {
+template <int K, int U, int MODE, class F>
+__device__ F wave_term_class(const Span<F>& a, int64_t first, int64_t step, int lane) {
   bool nan = false, pinf = false, ninf = false;
   for (int64_t t = first; t < a.tiles; t += step) {
-    double vx[U], vy[U];
-    load_tile<T, U, MODE>(a, t, lane, vx, vy);
+    F vx[U], vy[U];
+    load_tile<K, U, MODE>(a, t, lane, vx, vy);
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      const double v = plain_term<T>(vx[k], T == kDot ? vy[k] : 0.0);
+      const F v = plain_term<K>(vx[k], K == kDot ? vy[k] : F(0));
       nan |= (v != v);
-      pinf |= (v == __builtin_inf());
-      ninf |= (v == -__builtin_inf());
+      pinf |= (v == (F)__builtin_inf());
+      ninf |= (v == -(F)__builtin_inf());
     }
   }
   const bool any_nan = __any(nan), any_p = __any(pinf), any_n = __any(ninf);
-  if (any_nan || (any_p && any_n) || !(any_p || any_n)) return __builtin_nan("");
-  return any_p ? __builtin_inf() : -__builtin_inf();
+  if (any_nan || (any_p && any_n) || !(any_p || any_n)) return (F)__builtin_nan("");
+  return any_p ? (F)__builtin_inf() : -(F)__builtin_inf();
 }
 
 // Per-wave running state of wave_accumulate.
+template <class F>
 struct WaveRun {
-  int idx;     // common index of the wave's bins
-  double lim;  // admit_limit(idx)
-  double scl;  // nrm2: current scale of the wave
-  double inv;  // nrm2: 1 / scl
-  int dep;     // deposits since the last renormalisation
+  int idx;  // common index of the wave's bins
+  F lim;    // admit_limit(idx)
+  F scl;    // nrm2: current scale of the wave
+  F inv;    // nrm2: 1 / scl
+  int dep;  // deposits since the last renormalisation
 };
 
 // Deposits one tile of raw terms (cx, cy) into the lane's bins.
-template <int CAP, int T, int U>
-__device__ __forceinline__ void deposit_tile(int fold, const double (&cx)[U], const double (&cy)[U], WaveRun& w,
-                                             State<CAP>& s) {
-  double v[U];
-  double m = 0.0;
+template <int CAP, int K, int U, class F>
+__device__ __forceinline__ void deposit_tile(int fold, const F (&cx)[U], const F (&cy)[U], WaveRun<F>& w,
+                                             State<CAP, F>& s) {
+  F v[U];
+  F m = F(0);
 #pragma unroll
   for (int k = 0; k < U; ++k) {
-    if (T == kSum) v[k] = cx[k];
-    if (T == kASum) v[k] = fabs(cx[k]);
-    if (T == kDot) v[k] = cx[k] * cy[k];
-    if (T == kSSq) v[k] = cx[k];
+    if (K == kSum) v[k] = cx[k];
+    if (K == kASum) v[k] = fabs(cx[k]);
+    if (K == kDot) v[k] = cx[k] * cy[k];
+    if (K == kSSq) v[k] = cx[k];
     m = fmax(m, fabs(v[k]));
   }
   bool need;
-  if (T == kSSq) {
-    const double ms = m * w.inv;
+  if (K == kSSq) {
+    const F ms = m * w.inv;
     need = (dscale(m) > w.scl) || (ms * ms >= w.lim);
   } else {
     need = (m >= w.lim);
   }
   // the endurance renorm comes before the update as in the reference drivers
-  if (w.dep + U > kEndurance) {
+  if (w.dep + U > Fmt<F>::endurance) {
     renorm(fold, s.p, s.c);
     w.dep = 0;
   }
   if (__any(need)) {
-    double M = uniform_double(wave_max(m));
-    if (T == kSSq) {
-      const double ns = dscale(M);
+    F M = uniform_value(wave_max(m));
+    if (K == kSSq) {
+      const F ns = dscale(M);
       if (ns > w.scl) {
-        if (w.scl > 0.0) {
+        if (w.scl > F(0)) {
           rescale(fold, ns, w.scl, s.p, s.c);
           w.idx = uniform_int(dmindex(s.p[0]));
         }
         w.scl = ns;
-        w.inv = 1.0 / w.scl;
+        w.inv = F(1) / w.scl;  // exact: scales are powers of two
       }
       M = M / w.scl;  // reference dmdssq.ccog:98-99
       M = M * M;
@@ -298,50 +319,50 @@ __device__ __forceinline__ void deposit_tile(int fold, const double (&cx)[U], co
       raise_index(fold, w.idx - nI, nI, s.p, s.c);
       w.idx = nI;
     }
-    w.lim = admit_limit(w.idx);
+    w.lim = admit_limit<F>(w.idx);
   }
   if (w.idx == 0) {
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      double tv = v[k];
-      if (T == kSSq) { tv = v[k] * w.inv; tv = tv * tv; }
+      F tv = v[k];
+      if (K == kSSq) { tv = v[k] * w.inv; tv = tv * tv; }
       deposit(fold, true, tv, s.p);
     }
   } else {
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      double tv = v[k];
-      if (T == kSSq) { tv = v[k] * w.inv; tv = tv * tv; }
+      F tv = v[k];
+      if (K == kSSq) { tv = v[k] * w.inv; tv = tv * tv; }
       deposit(fold, false, tv, s.p);
     }
   }
   w.dep += U;
 }
 
-// Fresh wave state: every wave starts at index 51, the reference's
+// Fresh wave state: every wave starts at the last index, the reference's
 // representation of an empty sum (dmdupdate of 0.0 on a zero value), so every
 // tile deposits.
-template <int CAP>
-__device__ __forceinline__ void wave_start(int fold, WaveRun& w, State<CAP>& s) {
-  w.idx = kMaxIndex;
+template <int CAP, class F>
+__device__ __forceinline__ void wave_start(int fold, WaveRun<F>& w, State<CAP, F>& s) {
+  w.idx = Fmt<F>::max_index;
   set_bins(fold, w.idx, s.p, s.c);
-  w.lim = admit_limit(w.idx);
-  w.scl = 0.0;
-  w.inv = 0.0;
+  w.lim = admit_limit<F>(w.idx);
+  w.scl = F(0);
+  w.inv = F(0);
   w.dep = 0;
 }
 
 // Ends a wave's reduction of the tiles t0, t0 + tstep, ... of `a`: renorm,
 // exact same-index butterfly over the lanes (they share the index and scale),
 // and, if the value is poisoned, the class of the terms.
-template <int CAP, int T, int U, int MODE>
-__device__ __forceinline__ void wave_finish(const Span& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                            const WaveRun& w, State<CAP>& s) {
+template <int CAP, int K, int U, int MODE, class F>
+__device__ __forceinline__ void wave_finish(const Span<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
+                                            const WaveRun<F>& w, State<CAP, F>& s) {
   renorm(fold, s.p, s.c);
   s.scl = w.scl;
 #pragma unroll
   for (int st = 0; st < 6; ++st) {
-    double op[CAP], oc[CAP];
+    F op[CAP], oc[CAP];
 RB_UNROLL_CAP
     for (int j = 0; j < CAP; ++j) {
       op[j] = lane_partner(s.p[j], st);
@@ -353,37 +374,37 @@ RB_UNROLL_CAP
       s.p[0] = s.p[0] + op[0];  // keep the poison visible
     }
   }
-  if (nan_or_inf(uniform_double(s.p[0]))) {
-    s.p[0] = wave_term_class<T, U, MODE>(a, t0, tstep, lane);
+  if (nan_or_inf(uniform_value(s.p[0]))) {
+    s.p[0] = wave_term_class<K, U, MODE>(a, t0, tstep, lane);
   }
 }
 
 // One wave reduces the tiles t0, t0 + tstep, ... of a span into `s` (every
 // lane ends with the wave's value; a poisoned value is replaced by the class
 // of the terms).
-template <int CAP, int T, int U, int MODE>
-__device__ __forceinline__ void wave_accumulate(const Span& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                                State<CAP>& s) {
-  WaveRun w;
+template <int CAP, int K, int U, int MODE, class F>
+__device__ __forceinline__ void wave_accumulate(const Span<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
+                                                State<CAP, F>& s) {
+  WaveRun<F> w;
   wave_start(fold, w, s);
 
   // Two tile buffers in turn: the loads of the next tile are in flight while
   // the current one is deposited, and no buffer is ever copied.
-  double ax[U], ay[U], bx[U], by[U];
+  F ax[U], ay[U], bx[U], by[U];
   int64_t t = t0;
-  if (t < a.tiles) load_tile<T, U, MODE>(a, t, lane, ax, ay);
+  if (t < a.tiles) load_tile<K, U, MODE>(a, t, lane, ax, ay);
   while (t < a.tiles) {
     int64_t tn = t + tstep;
-    if (tn < a.tiles) load_tile<T, U, MODE>(a, tn, lane, bx, by);
-    deposit_tile<CAP, T, U>(fold, ax, ay, w, s);
+    if (tn < a.tiles) load_tile<K, U, MODE>(a, tn, lane, bx, by);
+    deposit_tile<CAP, K, U>(fold, ax, ay, w, s);
     if (tn >= a.tiles) break;
     t = tn;
     tn = t + tstep;
-    if (tn < a.tiles) load_tile<T, U, MODE>(a, tn, lane, ax, ay);
-    deposit_tile<CAP, T, U>(fold, bx, by, w, s);
+    if (tn < a.tiles) load_tile<K, U, MODE>(a, tn, lane, ax, ay);
+    deposit_tile<CAP, K, U>(fold, bx, by, w, s);
     t = tn;
   }
-  wave_finish<CAP, T, U, MODE>(a, fold, t0, tstep, lane, w, s);
+  wave_finish<CAP, K, U, MODE>(a, fold, t0, tstep, lane, w, s);
 }
 
 }  // namespace
