@@ -8,7 +8,8 @@ e ~ U{-40..40}), both operands resident in HBM; GB/s of algorithmic traffic
 Also reported under "extra": dsum 2^20 (uniform), dsum / dnrm2 2^28, ddot and
 dnrm2 2^28 at folds 2, 3, 4 and dsum with incX = 2, dgemv 16384^2 (N and T),
 dgemm 4096^3 (GFLOP/s and fraction of the FP64 VALU issue rate), the ragged
-batch and the MLP step, each as one timed configuration.
+batch and the MLP step, and the single-precision path (sdot / ssum / snrm2
+2^28 floats, sgemv 16384^2, sgemm 4096^3), each as one timed configuration.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -29,6 +30,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # FP64 VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz (one op per lane per
 # clock; the 78.6 TFLOP/s figure counts an FMA as two)
 VALU_PEAK_OPS = 256 * 64 * 2.4e9
+# FP32 VALU issue rate: 256 CUs x 128 lanes x 2.4 GHz (the published 157.3
+# TFLOP/s FP32 vector figure, an FMA counted as two)
+VALU_PEAK_OPS_FP32 = 256 * 128 * 2.4e9
 METRIC = "bit-exact ddot GB/s"
 
 
@@ -149,7 +153,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configurations)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", choices=["dsum_2p20", "dsum_2p28", "dnrm2_2p28", "folds", "dgemv", "dgemm", "ragged_batch",
-                                       "mlp"],
+                                       "mlp", "single"],
                     help="run one extra configuration alone and print its record (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy and a plain torch sum of 2^28 doubles (scales the HBM counters)")
@@ -332,7 +336,7 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
         C = torch.zeros(k * k, device=dev, dtype=torch.float64)
         fn = lambda: L.reproblas_hip_rdgemm_device(3, b"C", b"N", b"N", k, k, k, 1.0, vp(A), k, vp(B), k, 0.0, vp(C),
                                                    k, sp)
-        t = time_launches(fn, 2, 1, stream)
+        t = time_launches(fn, 3, 2, stream)
         ops = 11.0 * k ** 3  # (4*fold-1) ops per product, the reference's model (bench_rdgemm.c:125-127)
         ex["dgemm"] = ex["dgemm_NN_4096"] = {"ms": round(t * 1e3, 2), "GFLOP/s": round(ops / t / 1e9, 1),
                                              "valu_frac": round(ops / t / VALU_PEAK_OPS, 4),
@@ -341,6 +345,8 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
             f"{ops / t / VALU_PEAK_OPS:.3f} of VALU")
         del A, B, C
         torch.cuda.empty_cache()
+    if want("single"):
+        ex["single"] = single(L, dev, stream, sp, vp, steps)
     if want("ragged_batch"):
         ex["ragged_batch"] = ragged_batch(L, dev, stream, sp, vp)
     if want("mlp"):
@@ -351,6 +357,63 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
         ex.pop("dgemm", None)
         ex.pop("mlp", None)
     return ex
+
+
+def single(L, dev, stream, sp, vp, steps):
+    """The float path at fold 3: sdot / ssum / snrm2 over N = 2^28 wide-range
+    floats (x = s*m*2^e rounded to float, e in [-40,40]), sgemv 16384^2 (N and
+    T, beta 0.5) and sgemm 4096^3 NN."""
+    import torch
+
+    rec = {}
+    n = 1 << 28
+    x = wide_gpu(n, 11, dev).float()
+    y = wide_gpu(n, 12, dev).float()
+    torch.cuda.empty_cache()
+    out = torch.zeros(8, dtype=torch.float32, device=dev)
+    for name, fn, byts in (
+            ("sdot", lambda: L.reproblas_hip_rsdot_device(3, n, vp(x), 1, vp(y), 1, vp(out), sp), 8 * n),
+            ("ssum", lambda: L.reproblas_hip_rssum_device(3, n, vp(x), 1, vp(out), sp), 4 * n),
+            ("snrm2", lambda: L.reproblas_hip_rsnrm2_device(3, n, vp(x), 1, vp(out), sp), 4 * n)):
+        t = time_launches(fn, max(steps, 10), 20, stream)
+        rec[name + "_2p28"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
+                               "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] {name} 2^28: {t * 1e3:.3f} ms, {byts / t / 1e9:.1f} GB/s")
+    del x, y
+    torch.cuda.empty_cache()
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    m = 16384
+    A = torch.randn(m * m, generator=g, device=dev, dtype=torch.float32)
+    xv = torch.randn(m, generator=g, device=dev, dtype=torch.float32)
+    yv = torch.randn(m, generator=g, device=dev, dtype=torch.float32)
+    for tr in ("N", "T"):
+        fn = lambda: L.reproblas_hip_rsgemv_device(3, b"C", tr.encode(), m, m, 1.0, vp(A), m, vp(xv), 1, 0.5,
+                                                   vp(yv), 1, sp)
+        t = time_launches(fn, max(steps, 10), 10, stream)
+        byts = 4 * (m * m + 3 * m)
+        rec[f"sgemv_{tr}_16384"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
+                                    "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] sgemv {tr}: {t * 1e3:.3f} ms, {byts / t / 1e9:.1f} GB/s")
+    del A
+    torch.cuda.empty_cache()
+    k = 4096
+    A = torch.randn(k * k, generator=g, device=dev, dtype=torch.float32)
+    B = torch.randn(k * k, generator=g, device=dev, dtype=torch.float32)
+    C = torch.zeros(k * k, device=dev, dtype=torch.float32)
+    fn = lambda: L.reproblas_hip_rsgemm_device(3, b"C", b"N", b"N", k, k, k, 1.0, vp(A), k, vp(B), k, 0.0, vp(C), k,
+                                               sp)
+    t = time_launches(fn, 3, 2, stream)
+    ops = 11.0 * k ** 3
+    rec["sgemm_NN_4096"] = {"ms": round(t * 1e3, 2), "GFLOP/s": round(ops / t / 1e9, 1),
+                            "valu_frac": round(ops / t / VALU_PEAK_OPS_FP32, 4)}
+    log(f"[bench] sgemm: {t * 1e3:.2f} ms, {ops / t / 1e9:.1f} GFLOP/s (model ops)")
+    del A, B, C
+    torch.cuda.empty_cache()
+    import reproblas_amd as rb
+
+    rb._check()
+    return rec
 
 
 def ragged_batch(L, dev, stream, sp, vp):

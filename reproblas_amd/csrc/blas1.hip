@@ -173,7 +173,7 @@ bool launch_mode(int mode, const Blas1Args<F>& a, int blocks, hipStream_t s) {
 // Terms per lane per tile (double): 16 for the one-operand reductions, 8 for
 // dot, so every tile moves 8 KiB per wave; float tiles hold twice as many.
 //
-// Grid: at most kBlocksPerCU workgroups (4 waves) per CU for unit stride. On
+// Grid: at most RB_BLAS1_BLOCKS_PER_CU workgroups (4 waves) per CU for unit stride. On
 // MI355X a 2^28-term sum streams at ~6.7 TB/s with one workgroup per CU and
 // drops to ~5.3 TB/s with eight: every wave keeps its next tile in flight, and
 // fewer concurrent streams keep a narrower window of HBM pages open.
@@ -197,17 +197,29 @@ bool launch_mode(int mode, const Blas1Args<F>& a, int blocks, hipStream_t s) {
 #ifndef RB_BLAS1_MIN_TERMS_PER_WAVE
 #define RB_BLAS1_MIN_TERMS_PER_WAVE 2048
 #endif
+// Single precision: terms per lane per tile and workgroups per CU of the
+// float kernels (defaults: the same 8 KiB wave tiles as double).
+#ifndef RB_BLAS1_U_ONE_S
+#define RB_BLAS1_U_ONE_S 32
+#endif
+#ifndef RB_BLAS1_U_DOT_S
+#define RB_BLAS1_U_DOT_S 16
+#endif
+#ifndef RB_BLAS1_BLOCKS_PER_CU_S
+#define RB_BLAS1_BLOCKS_PER_CU_S 1
+#endif
 constexpr int kMinTermsPerWave = RB_BLAS1_MIN_TERMS_PER_WAVE;
 constexpr int kUOne = RB_BLAS1_U_ONE;
 constexpr int kUDot = RB_BLAS1_U_DOT;
-constexpr int kBlocksPerCU = RB_BLAS1_BLOCKS_PER_CU;
 constexpr int kBlocksPerCUStrided = RB_BLAS1_BLOCKS_PER_CU_STRIDED;
 
-// Terms per lane per tile for element type F: 8 KiB per wave tile.
+// Terms per lane per tile and unit-stride workgroups per CU for element type F.
 template <class F>
-constexpr int u_one() { return kUOne * (int)(sizeof(double) / sizeof(F)); }
+constexpr int u_one() { return sizeof(F) == sizeof(double) ? kUOne : RB_BLAS1_U_ONE_S; }
 template <class F>
-constexpr int u_dot() { return kUDot * (int)(sizeof(double) / sizeof(F)); }
+constexpr int u_dot() { return sizeof(F) == sizeof(double) ? kUDot : RB_BLAS1_U_DOT_S; }
+template <class F>
+constexpr int blocks_per_cu() { return sizeof(F) == sizeof(double) ? RB_BLAS1_BLOCKS_PER_CU : RB_BLAS1_BLOCKS_PER_CU_S; }
 
 template <int CAP, bool FIXED, class F>
 bool launch_term(Term term, int mode, const Blas1Args<F>& a, int blocks, hipStream_t s) {
@@ -345,7 +357,7 @@ bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const 
   // grid: a few streaming waves per CU, at most one wave per tile
   const int64_t min_tiles = std::max<int64_t>(1, kMinTermsPerWave / per_tile);
   int64_t blocks = (tiles + kWaves * min_tiles - 1) / (kWaves * min_tiles);
-  const int64_t cap = (int64_t)dev->cus * (mode == 2 ? kBlocksPerCUStrided : kBlocksPerCU);
+  const int64_t cap = (int64_t)dev->cus * (mode == 2 ? kBlocksPerCUStrided : blocks_per_cu<F>());
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
   a.partials = (F*)workspace(s, kSlotPartials, sizeof(F) * (size_t)blocks * (2 * fold + 1));
