@@ -118,6 +118,38 @@ void orc_sbsgemm(int fold, char Order, char TransA, char TransB, int M, int N, i
 void orc_rsgemm(int fold, char Order, char TransA, char TransB, int M, int N, int K, float alpha, const float *A,
                 int lda, const float *B, int ldb, float beta, float *C, int ldc);
 
+
+/* ------------------------------------------------ complex (oracle_z.c) ----
+ * Complex binned values of fold F are R[4F]: F interleaved (re, im)
+ * primaries, then F interleaved carries. Complex scalars are R[2]. */
+#define ORC_ZDECL(R, z, d, rz, rd)                                                                            \
+  void orc_##z##b##z##badd(int fold, const R *X, R *Y);                                                     \
+  void orc_##z##b##z##conv(int fold, const R *X, R *Y);                                                     \
+  void orc_##z##z##bconv(int fold, const R *X, R *out);                                                     \
+  void orc_##z##b##z##sum(int fold, int N, const R *X, int incX, R *Y);                                     \
+  void orc_##z##b##z##dotu(int fold, int N, const R *X, int incX, const R *Y, int incY, R *Z);              \
+  void orc_##z##b##z##dotc(int fold, int N, const R *X, int incX, const R *Y, int incY, R *Z);              \
+  void orc_##d##b##z##asum(int fold, int N, const R *X, int incX, R *Y);                                    \
+  R orc_##d##b##z##ssq(int fold, int N, const R *X, int incX, R scaleY, R *Y);                              \
+  void orc_r##z##sum(int fold, int N, const R *X, int incX, R *sum);                                        \
+  void orc_r##z##dotu(int fold, int N, const R *X, int incX, const R *Y, int incY, R *dot);                 \
+  void orc_r##z##dotc(int fold, int N, const R *X, int incX, const R *Y, int incY, R *dot);                 \
+  R orc_##rd##asum(int fold, int N, const R *X, int incX);                                                  \
+  R orc_##rd##nrm2(int fold, int N, const R *X, int incX);                                                  \
+  void orc_##z##b##z##gemv(int fold, char Order, char TransA, int M, int N, const R *alpha, const R *A,     \
+                           int lda, const R *X, int incX, R *Y, int incY);                                  \
+  void orc_r##z##gemv(int fold, char Order, char TransA, int M, int N, const R *alpha, const R *A, int lda, \
+                      const R *X, int incX, const R *beta, R *Y, int incY);                                 \
+  void orc_##z##b##z##gemm(int fold, char Order, char TransA, char TransB, int M, int N, int K,             \
+                           const R *alpha, const R *A, int lda, const R *B, int ldb, R *C, int ldc);        \
+  void orc_r##z##gemm(int fold, char Order, char TransA, char TransB, int M, int N, int K, const R *alpha,  \
+                      const R *A, int lda, const R *B, int ldb, const R *beta, R *C, int ldc);              \
+  void orc_##z##amax(int N, const R *X, int incX, R *amax);                                                 \
+  void orc_##z##amaxm(int N, const R *X, int incX, const R *Y, int incY, R *amaxm);
+ORC_ZDECL(double, z, d, rz, rdz)
+ORC_ZDECL(float, c, s, rc, rsc)
+#undef ORC_ZDECL
+
 #ifdef __cplusplus
 }
 #endif

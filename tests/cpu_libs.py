@@ -261,6 +261,127 @@ def reference(kind="avx"):
     return _cache[key]
 
 
+class CpuBlasZ:
+    """The complex calls of one precision ("z": complex128 data as float64
+    pairs, "c": complex64 as float32 pairs) on interleaved numpy arrays."""
+
+    # key -> (oracle name, reference name) with {z} (z/c), {d} (d/s), {rd} (rdz/rsc)
+    NAMES = {
+        "rsum": ("orc_r{z}sum", "reproBLAS_r{z}sum_sub"),
+        "rdotu": ("orc_r{z}dotu", "reproBLAS_r{z}dotu_sub"),
+        "rdotc": ("orc_r{z}dotc", "reproBLAS_r{z}dotc_sub"),
+        "rasum": ("orc_{rd}asum", "reproBLAS_{rd}asum"),
+        "rnrm2": ("orc_{rd}nrm2", "reproBLAS_{rd}nrm2"),
+        "rgemv": ("orc_r{z}gemv", "reproBLAS_r{z}gemv"),
+        "rgemm": ("orc_r{z}gemm", "reproBLAS_r{z}gemm"),
+        "bsum": ("orc_{z}b{z}sum", "binnedBLAS_{z}b{z}sum"),
+        "bdotu": ("orc_{z}b{z}dotu", "binnedBLAS_{z}b{z}dotu"),
+        "bdotc": ("orc_{z}b{z}dotc", "binnedBLAS_{z}b{z}dotc"),
+        "basum": ("orc_{d}b{z}asum", "binnedBLAS_{d}b{z}asum"),
+        "bssq": ("orc_{d}b{z}ssq", "binnedBLAS_{d}b{z}ssq"),
+        "bgemv": ("orc_{z}b{z}gemv", "binnedBLAS_{z}b{z}gemv"),
+        "bgemm": ("orc_{z}b{z}gemm", "binnedBLAS_{z}b{z}gemm"),
+        "bbadd": ("orc_{z}b{z}badd", "binned_{z}b{z}badd"),
+        "bconv": ("orc_{z}b{z}conv", "binned_{z}b{z}conv"),
+        "zbconv": ("orc_{z}{z}bconv", "binned_{z}{z}bconv_sub"),
+        "amax": ("orc_{z}amax", "binnedBLAS_{z}amax_sub"),
+        "amaxm": ("orc_{z}amaxm", "binnedBLAS_{z}amaxm_sub"),
+    }
+
+    def __init__(self, path, style, prec):
+        self.path, self.style, self.prec = path, style, prec
+        self.lib = ctypes.CDLL(path)
+        self.dtype = np.float64 if prec == "z" else np.float32
+        R = _c_double if prec == "z" else _c_float
+        i, c, v = _c_int, _c_char, _vp
+        sig = {
+            "rsum": (None, [i, i, v, i, v]), "rdotu": (None, [i, i, v, i, v, i, v]),
+            "rdotc": (None, [i, i, v, i, v, i, v]), "rasum": (R, [i, i, v, i]), "rnrm2": (R, [i, i, v, i]),
+            "rgemv": (None, [i, c, c, i, i, v, v, i, v, i, v, v, i]),
+            "rgemm": (None, [i, c, c, c, i, i, i, v, v, i, v, i, v, v, i]),
+            "bsum": (None, [i, i, v, i, v]), "bdotu": (None, [i, i, v, i, v, i, v]),
+            "bdotc": (None, [i, i, v, i, v, i, v]), "basum": (None, [i, i, v, i, v]),
+            "bssq": (R, [i, i, v, i, R, v]),
+            "bgemv": (None, [i, c, c, i, i, v, v, i, v, i, v, i]),
+            "bgemm": (None, [i, c, c, c, i, i, i, v, v, i, v, i, v, i]),
+            "bbadd": (None, [i, v, v]), "bconv": (None, [i, v, v]), "zbconv": (None, [i, v, v]),
+            "amax": (None, [i, v, i, v]), "amaxm": (None, [i, v, i, v, i, v]),
+        }
+        sub = {"z": prec, "d": "d" if prec == "z" else "s", "rd": "rdz" if prec == "z" else "rsc"}
+        self.fn = {}
+        for k, (res, args) in sig.items():
+            name = self.NAMES[k][0 if style == "orc" else 1].format(**sub)
+            f = getattr(self.lib, name)
+            f.restype = res
+            f.argtypes = args
+            self.fn[k] = f
+
+    def p(self, a):
+        if not a.flags["C_CONTIGUOUS"] or a.dtype != self.dtype:
+            raise ValueError("complex checker calls need contiguous interleaved arrays of the precision's type")
+        return ctypes.c_void_p(a.ctypes.data)
+
+    def scalar(self, v):
+        return np.array([np.real(v), np.imag(v)], dtype=self.dtype)
+
+    # x, y: interleaved arrays of 2n reals
+    def rsum(self, fold, x, inc=1):
+        out = np.zeros(2, self.dtype)
+        self.fn["rsum"](fold, len(x) // (2 * abs(inc)), self.p(x), inc, self.p(out))
+        return out
+
+    def rdot(self, kind, fold, x, y):
+        out = np.zeros(2, self.dtype)
+        self.fn["rdot" + kind](fold, min(len(x), len(y)) // 2, self.p(x), 1, self.p(y), 1, self.p(out))
+        return out
+
+    def rasum(self, fold, x, inc=1):
+        return self.fn["rasum"](fold, len(x) // (2 * abs(inc)), self.p(x), inc)
+
+    def rnrm2(self, fold, x, inc=1):
+        return self.fn["rnrm2"](fold, len(x) // (2 * abs(inc)), self.p(x), inc)
+
+    def bsum(self, fold, x, Y):
+        self.fn["bsum"](fold, len(x) // 2, self.p(x), 1, self.p(Y))
+
+    def bdot(self, kind, fold, x, y, Z):
+        self.fn["bdot" + kind](fold, len(x) // 2, self.p(x), 1, self.p(y), 1, self.p(Z))
+
+    def basum(self, fold, x, Y):
+        self.fn["basum"](fold, len(x) // 2, self.p(x), 1, self.p(Y))
+
+    def bssq(self, fold, x, scale, Y):
+        return self.fn["bssq"](fold, len(x) // 2, self.p(x), 1, scale, self.p(Y))
+
+    def zbconv(self, fold, Y):
+        out = np.zeros(2, self.dtype)
+        self.fn["zbconv"](fold, self.p(Y), self.p(out))
+        return out
+
+    def bconv(self, fold, v):
+        Y = np.zeros(4 * fold, self.dtype)
+        sv = self.scalar(v)
+        self.fn["bconv"](fold, self.p(sv), self.p(Y))
+        return Y
+
+
+def oracle_z(prec="z"):
+    key = "orc_" + prec
+    if key not in _cache:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+        _cache[key] = CpuBlasZ(ORACLE_SO, "orc", prec)
+    return _cache[key]
+
+
+def reference_z(prec="z", kind="avx"):
+    key = f"ref_{prec}_{kind}"
+    if key not in _cache:
+        p = REF_SO[kind]
+        _cache[key] = CpuBlasZ(p, "ref", prec) if os.path.exists(p) else None
+    return _cache[key]
+
+
 def oracle_s():
     if "orc_s" not in _cache:
         if not os.path.exists(ORACLE_SO):
