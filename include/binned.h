@@ -27,6 +27,9 @@ extern "C" {
 
 /* reference include/binned.h:46 */
 typedef double double_binned;
+/* reference include/binned.h:55 -- double[4F]: F interleaved (re, im)
+ * primaries, then F interleaved carries */
+typedef double double_complex_binned;
 
 /* reference include/binned.h:84 -- bin width in bits */
 #define DBWIDTH 40
@@ -100,6 +103,8 @@ double binned_ufp(const double X);                              /* binned.h:377 
  * Single precision: float[2F] (F primaries, then F carries), folds 2..21. */
 /* reference include/binned.h:64 */
 typedef float float_binned;
+/* reference include/binned.h:73 -- float[4F], laid out as double_complex_binned */
+typedef float float_complex_binned;
 /* reference include/binned.h:95 -- bin width in bits */
 #define SBWIDTH 13
 /* reference include/binned.h:115 -- maximum index (20) */
@@ -167,6 +172,104 @@ float binned_smsmaddsq(const int fold, const float scaleX, const float *priX, co
 float binned_sbsbaddsq(const int fold, const float scaleX, const float_binned *X, const float scaleY,
                        float_binned *Y);                        /* binned.h:375 */
 float binned_ufpf(const float X);                               /* binned.h:378 */
+
+/* ------------------------------------------------------------ complex
+ * Each complex primitive applies the real one to both components (strides
+ * doubled), as the reference's do. X of the *add/*update/*deposit/*conv
+ * calls points to an interleaved (re, im) pair. */
+size_t binned_zbsize(const int fold);                               /* binned.h:220 */
+size_t binned_cbsize(const int fold);                               /* binned.h:222 */
+double_complex_binned *binned_zballoc(const int fold);              /* binned.h:225 */
+float_complex_binned *binned_cballoc(const int fold);               /* binned.h:227 */
+int binned_zbnum(const int fold);                                   /* binned.h:230 */
+int binned_cbnum(const int fold);                                   /* binned.h:232 */
+int binned_zmdenorm(const int fold, const double *priX);            /* binned.h:249 */
+int binned_cmdenorm(const int fold, const float *priX);             /* binned.h:251 */
+void binned_zbprint(const int fold, const double_complex_binned *X); /* binned.h:255 */
+void binned_zmprint(const int fold, const double *priX, const int incpriX, const double *carX,
+                    const int inccarX);                             /* binned.h:256 */
+void binned_cbprint(const int fold, const float_complex_binned *X); /* binned.h:259 */
+void binned_cmprint(const int fold, const float *priX, const int incpriX, const float *carX,
+                    const int inccarX);                             /* binned.h:260 */
+void binned_zbzbset(const int fold, const double_complex_binned *X, double_complex_binned *Y); /* binned.h:264 */
+void binned_zmzmset(const int fold, const double *priX, const int incpriX, const double *carX, const int inccarX,
+                    double *priY, const int incpriY, double *carY, const int inccarY); /* binned.h:265 */
+void binned_zbdbset(const int fold, const double_binned *X, double_complex_binned *Y); /* binned.h:266 */
+void binned_zmdmset(const int fold, const double *priX, const int incpriX, const double *carX, const int inccarX,
+                    double *priY, const int incpriY, double *carY, const int inccarY); /* binned.h:267 */
+void binned_cbcbset(const int fold, const float_complex_binned *X, float_complex_binned *Y); /* binned.h:270 */
+void binned_cmcmset(const int fold, const float *priX, const int incpriX, const float *carX, const int inccarX,
+                    float *priY, const int incpriY, float *carY, const int inccarY); /* binned.h:271 */
+void binned_cbsbset(const int fold, const float_binned *X, float_complex_binned *Y); /* binned.h:272 */
+void binned_cmsmset(const int fold, const float *priX, const int incpriX, const float *carX, const int inccarX,
+                    float *priY, const int incpriY, float *carY, const int inccarY); /* binned.h:273 */
+void binned_zbsetzero(const int fold, double_complex_binned *X);    /* binned.h:277 */
+void binned_zmsetzero(const int fold, double *priX, const int incpriX, double *carX,
+                      const int inccarX);                           /* binned.h:278 */
+void binned_cbsetzero(const int fold, float_complex_binned *X);     /* binned.h:281 */
+void binned_cmsetzero(const int fold, float *priX, const int incpriX, float *carX,
+                      const int inccarX);                           /* binned.h:282 */
+void binned_zbzbadd(const int fold, const double_complex_binned *X, double_complex_binned *Y); /* binned.h:286 */
+void binned_zmzmadd(const int fold, const double *priX, const int incpriX, const double *carX, const int inccarX,
+                    double *priY, const int incpriY, double *carY, const int inccarY); /* binned.h:287 */
+void binned_cbcbadd(const int fold, const float_complex_binned *X, float_complex_binned *Y); /* binned.h:290 */
+void binned_cmcmadd(const int fold, const float *priX, const int incpriX, const float *carX, const int inccarX,
+                    float *priY, const int incpriY, float *carY, const int inccarY); /* binned.h:291 */
+/* binned.h:294,296 -- host or device arrays (both operands of one kind) */
+void binned_zbzbaddv(const int fold, const int N, const double_complex_binned *X, const int incX,
+                     double_complex_binned *Y, const int incY);
+void binned_cbcbaddv(const int fold, const int N, const float_complex_binned *X, const int incX,
+                     float_complex_binned *Y, const int incY);
+void binned_zbzadd(const int fold, const void *X, double_complex_binned *Y); /* binned.h:300 */
+void binned_zmzadd(const int fold, const void *X, double *priY, const int incpriY, double *carY,
+                   const int inccarY);                              /* binned.h:301 */
+void binned_cbcadd(const int fold, const void *X, float_complex_binned *Y); /* binned.h:304 */
+void binned_cmcadd(const int fold, const void *X, float *priY, const int incpriY, float *carY,
+                   const int inccarY);                              /* binned.h:305 */
+void binned_zbzupdate(const int fold, const void *X, double_complex_binned *Y); /* binned.h:309 */
+void binned_zmzupdate(const int fold, const void *X, double *priY, const int incpriY, double *carY,
+                      const int inccarY);                           /* binned.h:310 */
+void binned_zbdupdate(const int fold, const double X, double_complex_binned *Y); /* binned.h:311 */
+void binned_zmdupdate(const int fold, const double X, double *priY, const int incpriY, double *carY,
+                      const int inccarY);                           /* binned.h:312 */
+void binned_cbcupdate(const int fold, const void *X, float_complex_binned *Y); /* binned.h:315 */
+void binned_cmcupdate(const int fold, const void *X, float *priY, const int incpriY, float *carY,
+                      const int inccarY);                           /* binned.h:316 */
+void binned_cbsupdate(const int fold, const float X, float_complex_binned *Y); /* binned.h:317 */
+void binned_cmsupdate(const int fold, const float X, float *priY, const int incpriY, float *carY,
+                      const int inccarY);                           /* binned.h:318 */
+void binned_zbzdeposit(const int fold, const void *X, double_complex_binned *Y); /* binned.h:322 */
+void binned_zmzdeposit(const int fold, const void *X, double *priY, const int incpriY); /* binned.h:323 */
+void binned_cbcdeposit(const int fold, const void *X, float_complex_binned *Y); /* binned.h:326 */
+void binned_cmcdeposit(const int fold, const void *X, float *priY, const int incpriY); /* binned.h:327 */
+void binned_zbrenorm(const int fold, double_complex_binned *X);     /* binned.h:331 */
+void binned_zmrenorm(const int fold, double *priX, const int incpriX, double *carX,
+                     const int inccarX);                            /* binned.h:332 */
+void binned_cbrenorm(const int fold, float_complex_binned *X);      /* binned.h:335 */
+void binned_cmrenorm(const int fold, float *priX, const int incpriX, float *carX,
+                     const int inccarX);                            /* binned.h:336 */
+void binned_zbzconv(const int fold, const void *X, double_complex_binned *Y); /* binned.h:340 */
+void binned_zmzconv(const int fold, const void *X, double *priY, const int incpriY, double *carY,
+                    const int inccarY);                             /* binned.h:341 */
+void binned_cbcconv(const int fold, const void *X, float_complex_binned *Y); /* binned.h:344 */
+void binned_cmcconv(const int fold, const void *X, float *priY, const int incpriY, float *carY,
+                    const int inccarY);                             /* binned.h:345 */
+void binned_zzbconv_sub(const int fold, const double_complex_binned *X, void *conv); /* binned.h:348 */
+void binned_zzmconv_sub(const int fold, const double *priX, const int incpriX, const double *carX,
+                        const int inccarX, void *conv);             /* binned.h:349 */
+void binned_ccbconv_sub(const int fold, const float_complex_binned *X, void *conv); /* binned.h:352 */
+void binned_ccmconv_sub(const int fold, const float *priX, const int incpriX, const float *carX,
+                        const int inccarX, void *conv);             /* binned.h:353 */
+void binned_zbnegate(const int fold, double_complex_binned *X);     /* binned.h:357 */
+void binned_zmnegate(const int fold, double *priX, const int incpriX, double *carX,
+                     const int inccarX);                            /* binned.h:358 */
+void binned_cbnegate(const int fold, float_complex_binned *X);      /* binned.h:361 */
+void binned_cmnegate(const int fold, float *priX, const int incpriX, float *carX,
+                     const int inccarX);                            /* binned.h:362 */
+void binned_zmdrescale(const int fold, const double X, const double scaleY, double *priY, const int incpriY,
+                       double *carY, const int inccarY);            /* binned.h:368 */
+void binned_cmsrescale(const int fold, const float X, const float scaleY, float *priY, const int incpriY,
+                       float *carY, const int inccarY);             /* binned.h:370 */
 
 #ifdef __cplusplus
 }

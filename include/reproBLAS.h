@@ -79,6 +79,62 @@ void reproBLAS_sgemm(const char Order, const char TransA, const char TransB, con
                      const float alpha, const float *A, const int lda, const float *B, const int ldb,
                      const float beta, float *C, const int ldc); /* reproBLAS.h:135 */
 
+/* complex: X, Y, A, B, C point to interleaved (real, imaginary) pairs of
+ * double (z) or float (c); alpha, beta, and the *_sub results are such pairs.
+ * Complex binned accumulators have independent indices per component. */
+void reproBLAS_rzsum_sub(const int fold, const int N, const void *X, int incX, void *sum);         /* :37 */
+double reproBLAS_rdzasum(const int fold, const int N, const void *X, const int incX);              /* :38 */
+double reproBLAS_rdznrm2(const int fold, const int N, const void *X, int incX);                    /* :39 */
+void reproBLAS_rzdotc_sub(const int fold, const int N, const void *X, const int incX, const void *Y,
+                          const int incY, void *dotc);                                            /* :40 */
+void reproBLAS_rzdotu_sub(const int fold, const int N, const void *X, const int incX, const void *Y,
+                          const int incY, void *dotu);                                            /* :41 */
+void reproBLAS_rcsum_sub(const int fold, const int N, const void *X, const int incX, void *sum);   /* :43 */
+float reproBLAS_rscasum(const int fold, const int N, const void *X, const int incX);               /* :44 */
+float reproBLAS_rscnrm2(const int fold, const int N, const void *X, const int incX);               /* :45 */
+void reproBLAS_rcdotc_sub(const int fold, const int N, const void *X, const int incX, const void *Y,
+                          const int incY, void *dotc);                                            /* :46 */
+void reproBLAS_rcdotu_sub(const int fold, const int N, const void *X, const int incX, const void *Y,
+                          const int incY, void *dotu);                                            /* :47 */
+void reproBLAS_rzgemv(const int fold, const char Order, const char TransA, const int M, const int N,
+                      const void *alpha, const void *A, const int lda, const void *X, const int incX,
+                      const void *beta, void *Y, const int incY); /* reproBLAS.h:73 */
+void reproBLAS_rzgemm(const int fold, const char Order, const char TransA, const char TransB, const int M,
+                      const int N, const int K, const void *alpha, const void *A, const int lda, const void *B,
+                      const int ldb, const void *beta, void *C, const int ldc); /* reproBLAS.h:78 */
+void reproBLAS_rcgemv(const int fold, const char Order, const char TransA, const int M, const int N,
+                      const void *alpha, const void *A, const int lda, const void *X, const int incX,
+                      const void *beta, void *Y, const int incY); /* reproBLAS.h:85 */
+void reproBLAS_rcgemm(const int fold, const char Order, const char TransA, const char TransB, const int M,
+                      const int N, const int K, const void *alpha, const void *A, const int lda, const void *B,
+                      const int ldb, const void *beta, void *C, const int ldc); /* reproBLAS.h:90 */
+void reproBLAS_zsum_sub(const int N, const void *X, int incX, void *sum);                          /* :107 */
+double reproBLAS_dzasum(const int N, const void *X, const int incX);                               /* :108 */
+double reproBLAS_dznrm2(const int N, const void *X, int incX);                                     /* :109 */
+void reproBLAS_zdotc_sub(const int N, const void *X, const int incX, const void *Y, const int incY,
+                         void *dotc);                                                             /* :110 */
+void reproBLAS_zdotu_sub(const int N, const void *X, const int incX, const void *Y, const int incY,
+                         void *dotu);                                                             /* :111 */
+void reproBLAS_csum_sub(const int N, const void *X, const int incX, void *sum);                    /* :113 */
+float reproBLAS_scasum(const int N, const void *X, const int incX);                                /* :114 */
+float reproBLAS_scnrm2(const int N, const void *X, const int incX);                                /* :115 */
+void reproBLAS_cdotc_sub(const int N, const void *X, const int incX, const void *Y, const int incY,
+                         void *dotc);                                                             /* :116 */
+void reproBLAS_cdotu_sub(const int N, const void *X, const int incX, const void *Y, const int incY,
+                         void *dotu);                                                             /* :117 */
+void reproBLAS_zgemv(const char Order, const char TransA, const int M, const int N, const void *alpha,
+                     const void *A, const int lda, const void *X, const int incX, const void *beta, void *Y,
+                     const int incY); /* reproBLAS.h:141 */
+void reproBLAS_zgemm(const char Order, const char TransA, const char TransB, const int M, const int N, const int K,
+                     const void *alpha, const void *A, const int lda, const void *B, const int ldb,
+                     const void *beta, void *C, const int ldc); /* reproBLAS.h:146 */
+void reproBLAS_cgemv(const char Order, const char TransA, const int M, const int N, const void *alpha,
+                     const void *A, const int lda, const void *X, const int incX, const void *beta, void *Y,
+                     const int incY); /* reproBLAS.h:152 */
+void reproBLAS_cgemm(const char Order, const char TransA, const char TransB, const int M, const int N, const int K,
+                     const void *alpha, const void *A, const int lda, const void *B, const int ldb,
+                     const void *beta, void *C, const int ldc); /* reproBLAS.h:157 */
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,38 @@ int reproblas_hip_rsgemm_device(int fold, char Order, char TransA, char TransB, 
                                 float alpha, const float *A, int64_t lda, const float *B, int64_t ldb, float beta,
                                 float *C, int64_t ldc, void *stream);
 
+/* Complex counterparts on device data (interleaved pairs, increments in
+ * complex elements): result holds two values for sum/dotu/dotc, one for
+ * asum/nrm2 (reproBLAS_rz* / rc* / rdz* / rsc* semantics). */
+int reproblas_hip_rzsum_device(int fold, int64_t N, const double *X, int64_t incX, double *result, void *stream);
+int reproblas_hip_rzdotu_device(int fold, int64_t N, const double *X, int64_t incX, const double *Y, int64_t incY,
+                                double *result, void *stream);
+int reproblas_hip_rzdotc_device(int fold, int64_t N, const double *X, int64_t incX, const double *Y, int64_t incY,
+                                double *result, void *stream);
+int reproblas_hip_rdzasum_device(int fold, int64_t N, const double *X, int64_t incX, double *result, void *stream);
+int reproblas_hip_rdznrm2_device(int fold, int64_t N, const double *X, int64_t incX, double *result, void *stream);
+int reproblas_hip_rcsum_device(int fold, int64_t N, const float *X, int64_t incX, float *result, void *stream);
+int reproblas_hip_rcdotu_device(int fold, int64_t N, const float *X, int64_t incX, const float *Y, int64_t incY,
+                                float *result, void *stream);
+int reproblas_hip_rcdotc_device(int fold, int64_t N, const float *X, int64_t incX, const float *Y, int64_t incY,
+                                float *result, void *stream);
+int reproblas_hip_rscasum_device(int fold, int64_t N, const float *X, int64_t incX, float *result, void *stream);
+int reproblas_hip_rscnrm2_device(int fold, int64_t N, const float *X, int64_t incX, float *result, void *stream);
+
+/* complex gemv / gemm on device operands; alpha and beta are host pairs */
+int reproblas_hip_rzgemv_device(int fold, char Order, char TransA, int64_t M, int64_t N, const double *alpha,
+                                const double *A, int64_t lda, const double *X, int64_t incX, const double *beta,
+                                double *Y, int64_t incY, void *stream);
+int reproblas_hip_rzgemm_device(int fold, char Order, char TransA, char TransB, int64_t M, int64_t N, int64_t K,
+                                const double *alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
+                                const double *beta, double *C, int64_t ldc, void *stream);
+int reproblas_hip_rcgemv_device(int fold, char Order, char TransA, int64_t M, int64_t N, const float *alpha,
+                                const float *A, int64_t lda, const float *X, int64_t incX, const float *beta,
+                                float *Y, int64_t incY, void *stream);
+int reproblas_hip_rcgemm_device(int fold, char Order, char TransA, char TransB, int64_t M, int64_t N, int64_t K,
+                                const float *alpha, const float *A, int64_t lda, const float *B, int64_t ldb,
+                                const float *beta, float *C, int64_t ldc, void *stream);
+
 /* gemm tiling (the autotuner's knob; every tiling gives identical bits).
  * Returns the number of tilings; set/get select by ordinal. */
 int reproblas_hip_gemm_tiling_count(void);

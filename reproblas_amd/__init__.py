@@ -27,6 +27,9 @@ __all__ = [
     "ssum", "sasum", "snrm2", "sdot", "sgemv", "sgemm",
     "sbssum", "sbsasum", "sbsdot", "sbsssq", "sbsgemv", "sbsgemm", "samax", "samaxm",
     "sbsconv", "ssbconv", "sbsbadd", "sbsbaddv", "sbsetzero", "sbnum",
+    "rzsum", "rzdotu", "rzdotc", "rdzasum", "rdznrm2", "rzgemv", "rzgemm",
+    "rcsum", "rcdotu", "rcdotc", "rscasum", "rscnrm2", "rcgemv", "rcgemm",
+    "zbzsum", "zbzdotu", "zbzdotc", "dbzasum", "dbzssq", "cbcsum", "cbcdotu", "cbcdotc", "sbcasum", "sbcssq",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -136,6 +139,35 @@ _SIGS = {
                                              _c_i64, _c_float, _vp, _c_i64, _vp]),
     "reproblas_hip_rsgemm_device": (_c_int, [_c_int, _c_char, _c_char, _c_char, _c_i64, _c_i64, _c_i64, _c_float,
                                              _vp, _c_i64, _vp, _c_i64, _c_float, _vp, _c_i64, _vp]),
+    # complex (reproBLAS.h, binnedBLAS.h, reproblas_hip.h); pointers to interleaved pairs
+    **{f"reproBLAS_r{z}sum_sub": (None, [_c_int, _c_int, _vp, _c_int, _vp]) for z in "zc"},
+    **{f"reproBLAS_r{z}dot{k}_sub": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp]) for z in "zc" for k in "uc"},
+    "reproBLAS_rdzasum": (_c_double, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rdznrm2": (_c_double, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rscasum": (_c_float, [_c_int, _c_int, _vp, _c_int]),
+    "reproBLAS_rscnrm2": (_c_float, [_c_int, _c_int, _vp, _c_int]),
+    **{f"reproBLAS_r{z}gemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp,
+                                      _vp, _c_int]) for z in "zc"},
+    **{f"reproBLAS_r{z}gemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _vp, _vp, _c_int,
+                                      _vp, _c_int, _vp, _vp, _c_int]) for z in "zc"},
+    **{f"binnedBLAS_{z}b{z}sum": (None, [_c_int, _c_int, _vp, _c_int, _vp]) for z in "zc"},
+    **{f"binnedBLAS_{z}b{z}dot{k}": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp]) for z in "zc" for k in "uc"},
+    "binnedBLAS_dbzasum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
+    "binnedBLAS_sbcasum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
+    "binnedBLAS_dbzssq": (_c_double, [_c_int, _c_int, _vp, _c_int, _c_double, _vp]),
+    "binnedBLAS_sbcssq": (_c_float, [_c_int, _c_int, _vp, _c_int, _c_float, _vp]),
+    **{f"binnedBLAS_{z}b{z}gemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int,
+                                          _vp, _c_int]) for z in "zc"},
+    **{f"binnedBLAS_{z}b{z}gemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _vp, _vp,
+                                          _c_int, _vp, _c_int, _vp, _c_int]) for z in "zc"},
+    **{f"reproblas_hip_r{z}{op}_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _vp])
+       for z, op in (("z", "sum"), ("dz", "asum"), ("dz", "nrm2"), ("c", "sum"), ("sc", "asum"), ("sc", "nrm2"))},
+    **{f"reproblas_hip_r{z}dot{k}_device": (_c_int, [_c_int, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp, _vp])
+       for z in "zc" for k in "uc"},
+    **{f"reproblas_hip_r{z}gemv_device": (_c_int, [_c_int, _c_char, _c_char, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp,
+                                                   _c_i64, _vp, _vp, _c_i64, _vp]) for z in "zc"},
+    **{f"reproblas_hip_r{z}gemm_device": (_c_int, [_c_int, _c_char, _c_char, _c_char, _c_i64, _c_i64, _c_i64, _vp,
+                                                   _vp, _c_i64, _vp, _c_i64, _vp, _vp, _c_i64, _vp]) for z in "zc"},
     "reproblas_hip_gemm_tiling_count": (_c_int, []),
     "reproblas_hip_gemm_set_tiling": (_c_int, [_c_int]),
     "reproblas_hip_gemm_get_tiling": (_c_int, []),
@@ -596,3 +628,216 @@ def sbsgemm(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, Cb, ldc
     lib().binnedBLAS_sbsgemm(fold, _ch(order), _ch(transa), _ch(transb), M, N, K, alpha, _sp(A), lda, _sp(B), ldb,
                              _sp(Cb), ldc)
     _check()
+
+
+# ============================================================ complex --
+# complex128 ("z", double_complex_binned) and complex64 ("c",
+# float_complex_binned) operands; increments count complex elements.
+# Reference include/reproBLAS.h:37-47,73-90 and binnedBLAS.h:39-128.
+def _zp(a, prec):
+    return _ptr(a, "complex128" if prec == "z" else "complex64")
+
+
+def _zres(prec):
+    import numpy as np
+    return np.zeros(1, np.complex128 if prec == "z" else np.complex64)
+
+
+def _zpair(v, prec):
+    import numpy as np
+    return np.array([v], dtype=np.complex128 if prec == "z" else np.complex64)
+
+
+def _rzsum(prec, x, fold, inc, n):
+    _bind_stream(x)
+    out = _zres(prec)
+    getattr(lib(), f"reproBLAS_r{prec}sum_sub")(fold, _n_of(x, inc) if n is None else n, _zp(x, prec), inc,
+                                                _ptr(out, out.dtype.name))
+    _check()
+    return complex(out[0])
+
+
+def _rzdot(prec, kind, x, y, fold, incx, incy, n):
+    _bind_stream(x, y)
+    if n is None:
+        n = min(_n_of(x, incx), _n_of(y, incy))
+    out = _zres(prec)
+    getattr(lib(), f"reproBLAS_r{prec}dot{kind}_sub")(fold, n, _zp(x, prec), incx, _zp(y, prec), incy,
+                                                      _ptr(out, out.dtype.name))
+    _check()
+    return complex(out[0])
+
+
+def rzsum(x, fold=DEFAULT_FOLD, inc=1, n=None):
+    """reproBLAS_rzsum_sub: reproducible complex sum (each component binned)."""
+    return _rzsum("z", x, fold, inc, n)
+
+
+def rzdotu(x, y, fold=DEFAULT_FOLD, incx=1, incy=1, n=None):
+    """reproBLAS_rzdotu_sub: sum of x_i*y_i."""
+    return _rzdot("z", "u", x, y, fold, incx, incy, n)
+
+
+def rzdotc(x, y, fold=DEFAULT_FOLD, incx=1, incy=1, n=None):
+    """reproBLAS_rzdotc_sub: sum of conj(x_i)*y_i."""
+    return _rzdot("z", "c", x, y, fold, incx, incy, n)
+
+
+def rdzasum(x, fold=DEFAULT_FOLD, inc=1, n=None):
+    """reproBLAS_rdzasum: sum of |re| + |im|."""
+    _bind_stream(x)
+    r = lib().reproBLAS_rdzasum(fold, _n_of(x, inc) if n is None else n, _zp(x, "z"), inc)
+    _check()
+    return r
+
+
+def rdznrm2(x, fold=DEFAULT_FOLD, inc=1, n=None):
+    _bind_stream(x)
+    r = lib().reproBLAS_rdznrm2(fold, _n_of(x, inc) if n is None else n, _zp(x, "z"), inc)
+    _check()
+    return r
+
+
+def rcsum(x, fold=SDEFAULT_FOLD, inc=1, n=None):
+    return _rzsum("c", x, fold, inc, n)
+
+
+def rcdotu(x, y, fold=SDEFAULT_FOLD, incx=1, incy=1, n=None):
+    return _rzdot("c", "u", x, y, fold, incx, incy, n)
+
+
+def rcdotc(x, y, fold=SDEFAULT_FOLD, incx=1, incy=1, n=None):
+    return _rzdot("c", "c", x, y, fold, incx, incy, n)
+
+
+def rscasum(x, fold=SDEFAULT_FOLD, inc=1, n=None):
+    _bind_stream(x)
+    r = lib().reproBLAS_rscasum(fold, _n_of(x, inc) if n is None else n, _zp(x, "c"), inc)
+    _check()
+    return r
+
+
+def rscnrm2(x, fold=SDEFAULT_FOLD, inc=1, n=None):
+    _bind_stream(x)
+    r = lib().reproBLAS_rscnrm2(fold, _n_of(x, inc) if n is None else n, _zp(x, "c"), inc)
+    _check()
+    return r
+
+
+def _zstate(prec, kind, Y, x, y, fold, inc):
+    _bind_stream(x, y, Y)
+    rdt = "float64" if prec == "z" else "float32"
+    L = lib()
+    if kind == "sum":
+        getattr(L, f"binnedBLAS_{prec}b{prec}sum")(fold, _n_of(x, inc), _zp(x, prec), inc, _ptr(Y, rdt))
+    elif kind in ("u", "c"):
+        n = min(_n_of(x, inc), _n_of(y, inc))
+        getattr(L, f"binnedBLAS_{prec}b{prec}dot{kind}")(fold, n, _zp(x, prec), inc, _zp(y, prec), inc, _ptr(Y, rdt))
+    else:
+        getattr(L, ("binnedBLAS_dbzasum" if prec == "z" else "binnedBLAS_sbcasum"))(fold, _n_of(x, inc),
+                                                                                   _zp(x, prec), inc, _ptr(Y, rdt))
+    _check()
+
+
+def zbzsum(Y, x, fold=DEFAULT_FOLD, inc=1):
+    """binnedBLAS_zbzsum: Y (4*fold float64 complex binned value) += sum(x)."""
+    _zstate("z", "sum", Y, x, None, fold, inc)
+
+
+def zbzdotu(Z, x, y, fold=DEFAULT_FOLD, inc=1):
+    _zstate("z", "u", Z, x, y, fold, inc)
+
+
+def zbzdotc(Z, x, y, fold=DEFAULT_FOLD, inc=1):
+    _zstate("z", "c", Z, x, y, fold, inc)
+
+
+def dbzasum(Y, x, fold=DEFAULT_FOLD, inc=1):
+    """binnedBLAS_dbzasum: Y (2*fold float64 binned value) += sum(|re| + |im|)."""
+    _zstate("z", "asum", Y, x, None, fold, inc)
+
+
+def dbzssq(Y, x, scale=0.0, fold=DEFAULT_FOLD, inc=1):
+    _bind_stream(x, Y)
+    r = lib().binnedBLAS_dbzssq(fold, _n_of(x, inc), _zp(x, "z"), inc, scale, _ptr(Y))
+    _check()
+    return r
+
+
+def cbcsum(Y, x, fold=SDEFAULT_FOLD, inc=1):
+    _zstate("c", "sum", Y, x, None, fold, inc)
+
+
+def cbcdotu(Z, x, y, fold=SDEFAULT_FOLD, inc=1):
+    _zstate("c", "u", Z, x, y, fold, inc)
+
+
+def cbcdotc(Z, x, y, fold=SDEFAULT_FOLD, inc=1):
+    _zstate("c", "c", Z, x, y, fold, inc)
+
+
+def sbcasum(Y, x, fold=SDEFAULT_FOLD, inc=1):
+    _zstate("c", "asum", Y, x, None, fold, inc)
+
+
+def sbcssq(Y, x, scale=0.0, fold=SDEFAULT_FOLD, inc=1):
+    _bind_stream(x, Y)
+    r = lib().binnedBLAS_sbcssq(fold, _n_of(x, inc), _zp(x, "c"), inc, scale, _ptr(Y, "float32"))
+    _check()
+    return r
+
+
+def _rzgemv(prec, trans, alpha, A, x, beta, y, fold, order, M, N, lda, incx, incy):
+    rm = order in ("r", "R")
+    if M is None or N is None or lda is None:
+        sh = tuple(A.shape)
+        M, N, lda = (sh[0], sh[1], sh[1]) if rm else (sh[1], sh[0], sh[1])
+    al, be = _zpair(alpha, prec), _zpair(beta, prec)
+    _bind_stream(A, x, y)
+    getattr(lib(), f"reproBLAS_r{prec}gemv")(fold, _ch(order), _ch(trans), M, N, _ptr(al, al.dtype.name),
+                                             _zp(A, prec), lda, _zp(x, prec), incx, _ptr(be, be.dtype.name),
+                                             _zp(y, prec), incy)
+    _check()
+
+
+def _rzgemm(prec, transa, transb, alpha, A, B, beta, C, fold, order, M, N, K, lda, ldb, ldc):
+    rm = order in ("r", "R")
+
+    def dims(X):
+        sh = tuple(X.shape)
+        return (sh[0], sh[1], sh[1]) if rm else (sh[1], sh[0], sh[1])
+
+    ra, ca, la = dims(A)
+    rb_, cb, lb = dims(B)
+    rc, cc, lc = dims(C)
+    na = transa in ("n", "N")
+    M = (ra if na else ca) if M is None else M
+    K = (ca if na else ra) if K is None else K
+    N = (cb if transb in ("n", "N") else rb_) if N is None else N
+    lda, ldb, ldc = lda or la, ldb or lb, ldc or lc
+    al, be = _zpair(alpha, prec), _zpair(beta, prec)
+    _bind_stream(A, B, C)
+    getattr(lib(), f"reproBLAS_r{prec}gemm")(fold, _ch(order), _ch(transa), _ch(transb), M, N, K,
+                                             _ptr(al, al.dtype.name), _zp(A, prec), lda, _zp(B, prec), ldb,
+                                             _ptr(be, be.dtype.name), _zp(C, prec), ldc)
+    _check()
+
+
+def rzgemv(trans, alpha, A, x, beta, y, fold=DEFAULT_FOLD, order="C", M=None, N=None, lda=None, incx=1, incy=1):
+    """reproBLAS_rzgemv on a 2-D complex128 A (shapes as for :func:`rdgemv`; trans 'N', 'T' or 'C')."""
+    _rzgemv("z", trans, alpha, A, x, beta, y, fold, order, M, N, lda, incx, incy)
+
+
+def rzgemm(transa, transb, alpha, A, B, beta, C, fold=DEFAULT_FOLD, order="C", M=None, N=None, K=None,
+           lda=None, ldb=None, ldc=None):
+    """reproBLAS_rzgemm on complex128 operands; C is updated in place."""
+    _rzgemm("z", transa, transb, alpha, A, B, beta, C, fold, order, M, N, K, lda, ldb, ldc)
+
+
+def rcgemv(trans, alpha, A, x, beta, y, fold=SDEFAULT_FOLD, order="C", M=None, N=None, lda=None, incx=1, incy=1):
+    _rzgemv("c", trans, alpha, A, x, beta, y, fold, order, M, N, lda, incx, incy)
+
+
+def rcgemm(transa, transb, alpha, A, B, beta, C, fold=SDEFAULT_FOLD, order="C", M=None, N=None, K=None,
+           lda=None, ldb=None, ldc=None):
+    _rzgemm("c", transa, transb, alpha, A, B, beta, C, fold, order, M, N, K, lda, ldb, ldc)

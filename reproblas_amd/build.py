@@ -22,11 +22,15 @@ SOURCES = [
     "host_binned.hip",
     "binned_vec.hip",
     "blas1.hip",
+    "complex.hip",
+    "complex_mat.hip",
     "amax.hip",
     "gemv.hip",
     "gemm.hip",
     "batch.hip",
     "api.hip",
+    "host_complex.hip",
+    "api_complex.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

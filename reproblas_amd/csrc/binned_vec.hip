@@ -83,6 +83,35 @@ RB_UNROLL_CAP
   v[i * incv] = convert(fold, p, c);
 }
 
+// Complex binned values: both components, interleaved (4*fold per value).
+template <int CAP, bool FIXED, class F>
+__global__ __launch_bounds__(256) void zaddv_kernel(int fold_rt, int64_t n, const F* X, int64_t incX, F* Y,
+                                                    int64_t incY) {
+  const int fold = FIXED ? CAP : fold_rt;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const F* x = X + i * incX * 4 * fold;
+  F* y = Y + i * incY * 4 * fold;
+  for (int ch = 0; ch < 2; ++ch) {
+    F px[CAP], cx[CAP], py[CAP], cy[CAP];
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        px[j] = x[2 * j + ch];
+        cx[j] = x[2 * fold + 2 * j + ch];
+        py[j] = y[2 * j + ch];
+        cy[j] = y[2 * fold + 2 * j + ch];
+      }
+    add(fold, py, cy, px, cx);
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        y[2 * j + ch] = py[j];
+        y[2 * fold + 2 * j + ch] = cy[j];
+      }
+  }
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 template <class F>
@@ -118,6 +147,20 @@ bool sbsbaddv(int fold, int64_t n, const float* X, int64_t incX, float* Y, int64
   if (!fold_ok<float>(fold)) return false;
   if (n <= 0) return true;
   RB_FOLD_DISPATCH(addv_kernel, float, n, X, incX, Y, incY);
+  return RB_HIP(hipGetLastError());
+}
+
+bool zaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int64_t incY, hipStream_t s) {
+  if (!fold_ok<double>(fold)) return false;
+  if (n <= 0) return true;
+  RB_FOLD_DISPATCH(zaddv_kernel, double, n, X, incX, Y, incY);
+  return RB_HIP(hipGetLastError());
+}
+
+bool zaddv(int fold, int64_t n, const float* X, int64_t incX, float* Y, int64_t incY, hipStream_t s) {
+  if (!fold_ok<float>(fold)) return false;
+  if (n <= 0) return true;
+  RB_FOLD_DISPATCH(zaddv_kernel, float, n, X, incX, Y, incY);
   return RB_HIP(hipGetLastError());
 }
 

@@ -105,7 +105,7 @@ RB_UNROLL_CAP
     }
   }
   if (!have) {
-    set_bins(fold, kMaxIndex, acc.p, acc.c);
+    set_bins(fold, Fmt<F>::max_index, acc.p, acc.c);
     acc.scl = 0.0;
   }
   wave_merge_general<CAP, T>(fold, acc);

@@ -1,0 +1,507 @@
+// complex.hip -- single-pass binned reductions over complex device vectors:
+// sum, dotu, dotc (complex binned results) and asum, the scaled sum of
+// squares behind nrm2 (real binned results), for double complex and float
+// complex data (interleaved real, imaginary pairs).
+//
+// Reference path: binnedBLAS_zmzsum (src/binnedBLAS/zmzsum.ccog:57-...),
+// zmzdotu / zmzdotc (zmzdotu.ccog:59-..., zmzdotc.ccog:59-...), dmzasum and
+// dmzssq (dmzasum.ccog, dmzssq.ccog) and their float-complex counterparts
+// (cmcsum.ccog, cmcdotu.ccog, cmcdotc.ccog, smcasum.ccog, smcssq.ccog),
+// converted by binned_zzbconv_sub / ddbconv.
+//
+// A complex binned value is two real binned values with their own indices:
+// the real component takes x_r (sum), x_r*y_r and -(x_i*y_i) (dotu), x_r*y_r
+// and x_i*y_i (dotc); the imaginary one x_i, x_i*y_r and x_r*y_i (dotu),
+// -(x_i*y_r) and x_r*y_i (dotc) (zmzdotu.ccog:10288-10296 and the matching
+// scalar section of zmzdotc.ccog). asum deposits |x_r| and |x_i|, nrm2 the
+// squares of x_r/scl and x_i/scl, into the two components of a complex
+// temporary that the reference adds into one real value at the end; here the
+// two channels merge exactly the same way. Each lane keeps both channels in
+// registers; each channel's index (and scale) is common to the wave, as in
+// the real kernels (wave_sum.hpp), and all merges are exact, so the bits do
+// not depend on the tiling.
+#include <float.h>
+
+#include <algorithm>
+
+#include "binned_core.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+#include "wave_sum.hpp"
+
+namespace rb {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+template <class F>
+struct ZArgs {
+  int64_t n;  // complex elements
+  const F* x;
+  int64_t incx;  // in complex elements
+  const F* y;
+  int64_t incy;
+  F* partials;  // gridDim.x records of 2*(2*fold+1) elements
+  unsigned* ticket;
+  const F* state_in;  // complex binned (sum/dot) or real binned (asum/ssq)
+  F scale_in;
+  const F* scale_in_dev;
+  F* state_out;
+  F* scale_out;
+  F* result;  // 2 elements (sum/dot) or 1 (asum/nrm2)
+  int fold;
+};
+
+template <class F>
+struct ZSpan {
+  const F* x;
+  const F* y;
+  int64_t incx, incy, n, tiles, full_tiles;
+};
+
+constexpr bool is_dot(int T) { return T == kZDotU || T == kZDotC; }
+// merge rule of a channel
+constexpr int chan_kind(int T) { return T == kZSSq ? kSSq : kSum; }
+// deposit rule of a channel's raw terms
+constexpr int dep_kind(int T) { return T == kZASum ? kASum : (T == kZSSq ? kSSq : kSum); }
+// terms per lane per tile and channel
+constexpr int nterms(int T, int U) { return is_dot(T) ? 2 * U : U; }
+
+template <int U, class F>
+__device__ __forceinline__ ZSpan<F> make_zspan(const F* x, int64_t incx, const F* y, int64_t incy, int64_t n) {
+  ZSpan<F> s;
+  s.x = x;
+  s.y = y;
+  s.incx = incx;
+  s.incy = incy;
+  s.n = n;
+  s.tiles = (n + 64 * U - 1) / (64 * U);
+  s.full_tiles = n / (64 * U);
+  return s;
+}
+
+// The U complex elements of tile t for this lane. MODE 0: unit stride,
+// 16-byte aligned: 16-byte non-temporal loads (one double complex or two
+// float complex elements each, lanes on consecutive addresses); MODE 1: unit
+// stride, element loads; MODE 2: any stride. Out-of-range elements read as 0.
+template <int T, int U, int MODE, class F>
+__device__ __forceinline__ void load_ztile(const ZSpan<F>& a, int64_t t, int lane, F (&xr)[U], F (&xi)[U],
+                                           F (&yr)[U], F (&yi)[U]) {
+  const int64_t base = t * (64 * U);
+  const bool full = t < a.full_tiles;
+  if (MODE == 0 && full) {
+    if constexpr (sizeof(F) == 8) {
+      const d2* px = reinterpret_cast<const d2*>(a.x + 2 * base);
+      const d2* py = reinterpret_cast<const d2*>(a.y + 2 * base);
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const d2 v = __builtin_nontemporal_load(px + k * 64 + lane);
+        xr[k] = v.x;
+        xi[k] = v.y;
+        if (is_dot(T)) {
+          const d2 w = __builtin_nontemporal_load(py + k * 64 + lane);
+          yr[k] = w.x;
+          yi[k] = w.y;
+        }
+      }
+    } else {
+      const f4* px = reinterpret_cast<const f4*>(a.x + 2 * base);
+      const f4* py = reinterpret_cast<const f4*>(a.y + 2 * base);
+#pragma unroll
+      for (int k = 0; k < U / 2; ++k) {
+        const f4 v = __builtin_nontemporal_load(px + k * 64 + lane);
+        xr[2 * k] = v.x;
+        xi[2 * k] = v.y;
+        xr[2 * k + 1] = v.z;
+        xi[2 * k + 1] = v.w;
+        if (is_dot(T)) {
+          const f4 w = __builtin_nontemporal_load(py + k * 64 + lane);
+          yr[2 * k] = w.x;
+          yi[2 * k] = w.y;
+          yr[2 * k + 1] = w.z;
+          yi[2 * k + 1] = w.w;
+        }
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const int64_t e = base + k * 64 + lane;
+    const bool in = full || e < a.n;
+    const int64_t ix = (MODE == 2) ? e * a.incx : e;
+    xr[k] = in ? a.x[2 * ix] : F(0);
+    xi[k] = in ? a.x[2 * ix + 1] : F(0);
+    if (is_dot(T)) {
+      const int64_t iy = (MODE == 2) ? e * a.incy : e;
+      yr[k] = in ? a.y[2 * iy] : F(0);
+      yi[k] = in ? a.y[2 * iy + 1] : F(0);
+    }
+  }
+}
+
+// The terms of one channel (raw values for asum/ssq, whose deposit applies
+// |.| or the scaled square).
+template <int T, int U, class F>
+__device__ __forceinline__ void channel_terms(int ch, const F (&xr)[U], const F (&xi)[U], const F (&yr)[U],
+                                              const F (&yi)[U], F (&t)[nterms(T, U)]) {
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    if (!is_dot(T)) {
+      t[k] = ch == 0 ? xr[k] : xi[k];
+    } else if (T == kZDotU) {
+      if (ch == 0) {
+        t[k] = xr[k] * yr[k];
+        t[U + k] = (xi[k] * yi[k]) * F(-1);
+      } else {
+        t[k] = xi[k] * yr[k];
+        t[U + k] = xr[k] * yi[k];
+      }
+    } else {
+      if (ch == 0) {
+        t[k] = xr[k] * yr[k];
+        t[U + k] = xi[k] * yi[k];
+      } else {
+        t[k] = (xi[k] * yr[k]) * F(-1);
+        t[U + k] = xr[k] * yi[k];
+      }
+    }
+  }
+}
+
+// Class of a channel's terms over the tiles this wave walked (NaN if any
+// term is NaN or both infinities occur, else the infinity that occurs).
+template <int T, int U, int MODE, class F>
+__device__ F zwave_term_class(const ZSpan<F>& a, int64_t first, int64_t step, int lane, int ch) {
+  constexpr int NT = nterms(T, U);
+  bool nan = false, pinf = false, ninf = false;
+  for (int64_t t = first; t < a.tiles; t += step) {
+    F xr[U], xi[U], yr[U], yi[U], v[NT];
+    load_ztile<T, U, MODE>(a, t, lane, xr, xi, yr, yi);
+    channel_terms<T, U>(ch, xr, xi, yr, yi, v);
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const F u = (T == kZASum || T == kZSSq) ? fabs(v[k]) : v[k];
+      nan |= (u != u);
+      pinf |= (u == (F)__builtin_inf());
+      ninf |= (u == -(F)__builtin_inf());
+    }
+  }
+  const bool any_nan = __any(nan), any_p = __any(pinf), any_n = __any(ninf);
+  if (any_nan || (any_p && any_n) || !(any_p || any_n)) return (F)__builtin_nan("");
+  return any_p ? (F)__builtin_inf() : -(F)__builtin_inf();
+}
+
+template <int CAP, int T, int U, class F>
+__device__ __forceinline__ void deposit_ztile(int fold, const F (&xr)[U], const F (&xi)[U], const F (&yr)[U],
+                                              const F (&yi)[U], WaveRun<F> (&w)[2], State<CAP, F> (&s)[2]) {
+  constexpr int NT = nterms(T, U);
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    F t[NT];
+    channel_terms<T, U>(ch, xr, xi, yr, yi, t);
+    deposit_tile<CAP, dep_kind(T), NT>(fold, t, t, w[ch], s[ch]);
+  }
+}
+
+// One wave reduces the tiles t0, t0 + tstep, ... into both channels.
+template <int CAP, int T, int U, int MODE, class F>
+__device__ __forceinline__ void zwave_accumulate(const ZSpan<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
+                                                 State<CAP, F> (&s)[2]) {
+  WaveRun<F> w[2];
+  wave_start(fold, w[0], s[0]);
+  wave_start(fold, w[1], s[1]);
+  F axr[U], axi[U], ayr[U], ayi[U], bxr[U], bxi[U], byr[U], byi[U];
+  int64_t t = t0;
+  if (t < a.tiles) load_ztile<T, U, MODE>(a, t, lane, axr, axi, ayr, ayi);
+  while (t < a.tiles) {
+    int64_t tn = t + tstep;
+    if (tn < a.tiles) load_ztile<T, U, MODE>(a, tn, lane, bxr, bxi, byr, byi);
+    deposit_ztile<CAP, T, U>(fold, axr, axi, ayr, ayi, w, s);
+    if (tn >= a.tiles) break;
+    t = tn;
+    tn = t + tstep;
+    if (tn < a.tiles) load_ztile<T, U, MODE>(a, tn, lane, axr, axi, ayr, ayi);
+    deposit_ztile<CAP, T, U>(fold, bxr, bxi, byr, byi, w, s);
+    t = tn;
+  }
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    wave_butterfly(fold, w[ch], s[ch]);
+    if (nan_or_inf(uniform_value(s[ch].p[0]))) s[ch].p[0] = zwave_term_class<T, U, MODE>(a, t0, tstep, lane, ch);
+  }
+}
+
+template <int CAP, int T, class F>
+__device__ __forceinline__ void merge_wg(int fold, int wave, int lane, F (*lds)[2][2 * CAP + 1],
+                                         State<CAP, F> (&s)[2], State<CAP, F> (&acc)[2]) {
+  if (lane == 0) {
+    store_state(lds[wave][0], fold, s[0]);
+    store_state(lds[wave][1], fold, s[1]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      State<CAP, F> o;
+      load_state(lds[0][ch], fold, acc[ch]);
+      for (int w = 1; w < kWaves; ++w) {
+        load_state(lds[w][ch], fold, o);
+        merge<CAP, chan_kind(T)>(fold, acc[ch], o);
+      }
+    }
+  }
+}
+
+template <int CAP, bool FIXED, int T, int U, int MODE, class F>
+__global__ __launch_bounds__(kThreads) void zblas1_kernel(ZArgs<F> a) {
+  constexpr int KM = chan_kind(T);
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + wave;
+  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+  const ZSpan<F> sp = make_zspan<U>(a.x, a.incx, a.y, a.incy, a.n);
+  State<CAP, F> s[2];
+  zwave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s);
+
+  __shared__ F lds[kWaves][2][2 * CAP + 1];
+  __shared__ bool last;
+  const int rec = 2 * (2 * fold + 1);
+  {
+    State<CAP, F> acc[2];
+    merge_wg<CAP, T>(fold, wave, lane, lds, s, acc);
+    if (threadIdx.x == 0) {
+      store_state(a.partials + (int64_t)blockIdx.x * rec, fold, acc[0]);
+      store_state(a.partials + (int64_t)blockIdx.x * rec + (2 * fold + 1), fold, acc[1]);
+      __threadfence();
+      const unsigned ticket = atomicAdd(a.ticket, 1u);
+      last = (ticket == gridDim.x - 1);
+    }
+  }
+  __syncthreads();
+  if (!last) return;
+
+  // Last workgroup: merge every partial per channel, fold in the caller's
+  // value, convert.
+  __threadfence();
+  State<CAP, F> acc[2];
+  bool have = false;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      State<CAP, F> o;
+      const F* src = a.partials + (int64_t)b * rec + ch * (2 * fold + 1);
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          o.p[j] = __builtin_nontemporal_load(src + j);
+          o.c[j] = __builtin_nontemporal_load(src + fold + j);
+        }
+      o.scl = __builtin_nontemporal_load(src + 2 * fold);
+      if (!have)
+        acc[ch] = o;
+      else
+        merge<CAP, KM>(fold, acc[ch], o);
+    }
+    have = true;
+  }
+  if (!have) {
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      set_bins(fold, Fmt<F>::max_index, acc[ch].p, acc[ch].c);
+      acc[ch].scl = F(0);
+    }
+  }
+  wave_merge_general<CAP, KM>(fold, acc[0]);
+  wave_merge_general<CAP, KM>(fold, acc[1]);
+  __syncthreads();
+  State<CAP, F> tot[2];
+  merge_wg<CAP, T>(fold, wave, lane, lds, acc, tot);
+  if (threadIdx.x != 0) return;
+
+  if (is_dot(T) || T == kZSum) {
+    // complex result: each channel like the real kernel, interleaved layout
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+      State<CAP, F> y;
+      if (a.state_in) {
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) {
+            y.p[j] = a.state_in[2 * j + ch];
+            y.c[j] = a.state_in[2 * fold + 2 * j + ch];
+          }
+        renorm(fold, y.p, y.c);
+      } else {
+        set_zero(fold, y.p, y.c);
+      }
+      y.scl = F(0);
+      if (a.n == 0) {
+      } else if (y.p[0] == F(0)) {
+        y = tot[ch];
+      } else {
+        merge<CAP, kSum>(fold, y, tot[ch]);
+      }
+      renorm(fold, y.p, y.c);
+      if (a.state_out) {
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) {
+            a.state_out[2 * j + ch] = y.p[j];
+            a.state_out[2 * fold + 2 * j + ch] = y.c[j];
+          }
+      }
+      if (a.result) a.result[ch] = convert(fold, y.p, y.c);
+    }
+  } else {
+    // real result: the two channels are one sum (exact merge), then as the
+    // real kernel folds in the caller's value
+    State<CAP, F> v = tot[0];
+    merge<CAP, KM>(fold, v, tot[1]);
+    State<CAP, F> y;
+    if (a.state_in) {
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          y.p[j] = a.state_in[j];
+          y.c[j] = a.state_in[fold + j];
+        }
+      renorm(fold, y.p, y.c);
+    } else {
+      set_zero(fold, y.p, y.c);
+    }
+    y.scl = a.scale_in_dev ? *a.scale_in_dev : a.scale_in;
+    if (a.n == 0) {
+    } else if (y.p[0] == F(0)) {
+      y = v;
+    } else {
+      merge<CAP, KM>(fold, y, v);
+    }
+    renorm(fold, y.p, y.c);
+    F scale = y.scl;
+    if (T == kZSSq && nan_or_inf(y.p[0])) scale = F(1);
+    if (a.state_out) {
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          a.state_out[j] = y.p[j];
+          a.state_out[fold + j] = y.c[j];
+        }
+    }
+    if (a.scale_out) *a.scale_out = scale;
+    if (a.result) {
+      const F c = convert(fold, y.p, y.c);
+      *a.result = (T == kZSSq) ? (F)((double)scale * sqrt((double)c)) : c;
+    }
+  }
+  *a.ticket = 0u;
+}
+
+// Complex elements per lane per tile: 128 bytes of x per lane (64 of each
+// operand for the dots).
+template <int T, class F>
+constexpr int z_units() {
+  return (is_dot(T) ? 64 : 128) / (2 * (int)sizeof(F));
+}
+
+template <int CAP, bool FIXED, int T, class F>
+bool zlaunch_mode(int mode, const ZArgs<F>& a, int blocks, hipStream_t s) {
+  constexpr int U = z_units<T, F>();
+  switch (mode) {
+    case 0: zblas1_kernel<CAP, FIXED, T, U, 0, F><<<blocks, kThreads, 0, s>>>(a); break;
+    case 1: zblas1_kernel<CAP, FIXED, T, U, 1, F><<<blocks, kThreads, 0, s>>>(a); break;
+    default: zblas1_kernel<CAP, FIXED, T, U, 2, F><<<blocks, kThreads, 0, s>>>(a); break;
+  }
+  return RB_HIP(hipGetLastError());
+}
+
+template <int CAP, bool FIXED, class F>
+bool zlaunch_term(ZTerm term, int mode, const ZArgs<F>& a, int blocks, hipStream_t s) {
+  switch (term) {
+    case kZSum: return zlaunch_mode<CAP, FIXED, kZSum>(mode, a, blocks, s);
+    case kZDotU: return zlaunch_mode<CAP, FIXED, kZDotU>(mode, a, blocks, s);
+    case kZDotC: return zlaunch_mode<CAP, FIXED, kZDotC>(mode, a, blocks, s);
+    case kZASum: return zlaunch_mode<CAP, FIXED, kZASum>(mode, a, blocks, s);
+    default: return zlaunch_mode<CAP, FIXED, kZSSq>(mode, a, blocks, s);
+  }
+}
+
+int64_t zunits_of(ZTerm term, size_t esize) {
+  const bool dot = (term == kZDotU || term == kZDotC);
+  return (dot ? 64 : 128) / (2 * (int64_t)esize);
+}
+
+template <class F>
+bool zblas1_impl(int fold, ZTerm term, int64_t n, const F* x, int64_t incx, const F* y, int64_t incy,
+                 const F* state_in, F scale_in, const F* scale_in_dev, F* state_out, F* scale_out, F* result,
+                 hipStream_t s) {
+  if (fold < 2 || fold > Fmt<F>::max_fold) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, %d]", fold, Fmt<F>::max_fold);
+    return false;
+  }
+  const DeviceInfo* dev = device_info();
+  if (!dev) return false;
+  const bool dot = (term == kZDotU || term == kZDotC);
+  ZArgs<F> a;
+  a.n = n < 0 ? 0 : n;
+  a.x = x;
+  a.incx = incx;
+  a.y = y;
+  a.incy = incy;
+  a.state_in = state_in;
+  a.scale_in = scale_in;
+  a.scale_in_dev = scale_in_dev;
+  a.state_out = state_out;
+  a.scale_out = scale_out;
+  a.result = result;
+  a.fold = fold;
+  const int64_t per_tile = 64 * zunits_of(term, sizeof(F));
+  const int64_t tiles = (a.n + per_tile - 1) / per_tile;
+  int mode;
+  const bool unit = (incx == 1) && (!dot || incy == 1);
+  if (!unit) {
+    mode = 2;
+  } else {
+    const bool aligned = ((uintptr_t)x % 16 == 0) && (!dot || (uintptr_t)y % 16 == 0);
+    mode = aligned ? 0 : 1;
+  }
+  // grid: one workgroup per CU for unit stride (as the real kernels), four for
+  // strided operands; at least 2048 terms per wave for short vectors
+  const int64_t min_tiles = std::max<int64_t>(1, 2048 / (per_tile * (dot ? 2 : 1)));
+  int64_t blocks = (tiles + kWaves * min_tiles - 1) / (kWaves * min_tiles);
+  const int64_t cap = (int64_t)dev->cus * (mode == 2 ? 4 : 1);
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  a.partials = (F*)workspace(s, kSlotPartials, sizeof(F) * (size_t)blocks * 2 * (2 * fold + 1));
+  a.ticket = tickets(s, 1);
+  if (!a.partials || !a.ticket) return false;
+  switch (fold) {
+    case 2: return zlaunch_term<2, true>(term, mode, a, (int)blocks, s);
+    case 3: return zlaunch_term<3, true>(term, mode, a, (int)blocks, s);
+    case 4: return zlaunch_term<4, true>(term, mode, a, (int)blocks, s);
+    default:
+      if (fold <= 8) return zlaunch_term<8, false>(term, mode, a, (int)blocks, s);
+      return zlaunch_term<Fmt<F>::max_fold, false>(term, mode, a, (int)blocks, s);
+  }
+}
+
+}  // namespace
+
+bool zblas1(int fold, ZTerm term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
+            const double* state_in, double scale_in, const double* scale_in_dev, double* state_out,
+            double* scale_out, double* result, hipStream_t s) {
+  return zblas1_impl<double>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out,
+                             scale_out, result, s);
+}
+
+bool zblas1(int fold, ZTerm term, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy,
+            const float* state_in, float scale_in, const float* scale_in_dev, float* state_out, float* scale_out,
+            float* result, hipStream_t s) {
+  return zblas1_impl<float>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out,
+                            scale_out, result, s);
+}
+
+}  // namespace rb

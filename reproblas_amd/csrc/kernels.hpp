@@ -29,6 +29,21 @@ bool blas1_s(int fold, Term term, int64_t n, const float* x, int64_t incx, const
              const float* state_in, float scale_in, const float* scale_in_dev, float* state_out, float* scale_out,
              float* result, hipStream_t s);
 
+// Complex reductions over interleaved (real, imaginary) device vectors of n
+// complex elements (increments in complex elements):
+//   kZSum, kZDotU, kZDotC -- complex binned result: state_in/state_out hold
+//     4*fold values (interleaved primaries, then interleaved carries),
+//     result two values;
+//   kZASum, kZSSq -- real binned result (2*fold), result one value (kZSSq:
+//     scl*sqrt(conv)); scale_in/scale_in_dev/scale_out as for kSSq.
+enum ZTerm : int { kZSum = 0, kZDotU = 1, kZDotC = 2, kZASum = 3, kZSSq = 4 };
+bool zblas1(int fold, ZTerm term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
+            const double* state_in, double scale_in, const double* scale_in_dev, double* state_out,
+            double* scale_out, double* result, hipStream_t s);
+bool zblas1(int fold, ZTerm term, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy,
+            const float* state_in, float scale_in, const float* scale_in_dev, float* state_out, float* scale_out,
+            float* result, hipStream_t s);
+
 // Ragged batch of BLAS1 reductions over CSR segments of x (and y for dot),
 // unit stride. chunks[c] = [begin, end) element range (at most a few
 // thousand terms); segment s owns chunks [first_chunk[s], first_chunk[s+1]).
@@ -44,6 +59,9 @@ bool segments(int fold, Term term, const double* x, const double* y, const SegCh
 // dbdbaddv / sbsbaddv: Y[i] += X[i] for i < n (stride incX/incY in binned elements).
 bool dbdbaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int64_t incY, hipStream_t s);
 bool sbsbaddv(int fold, int64_t n, const float* X, int64_t incX, float* Y, int64_t incY, hipStream_t s);
+// zaddv: the same for complex binned values (4*fold each, both components).
+bool zaddv(int fold, int64_t n, const double* X, int64_t incX, double* Y, int64_t incY, hipStream_t s);
+bool zaddv(int fold, int64_t n, const float* X, int64_t incX, float* Y, int64_t incY, hipStream_t s);
 // conv_to_binned: Y[i] = dbdconv/sbsconv(beta == 1 ? y[i] : y[i]*beta) or zero when beta == 0.
 bool conv_to_binned(int fold, int64_t n, const double* y, int64_t incy, double beta, double* Yb, hipStream_t s);
 bool conv_to_binned(int fold, int64_t n, const float* y, int64_t incy, float beta, float* Yb, hipStream_t s);
@@ -83,6 +101,32 @@ bool rdgemv_device(int fold, char order, char trans, int64_t M, int64_t N, doubl
 // (reproBLAS_rdgemm) on device operands.
 bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, int64_t N, int64_t K, double alpha,
                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                   hipStream_t s);
+
+// Complex gemm / gemv on device operands (interleaved pairs; alpha and beta
+// are host pairs): binnedBLAS_zbzgemm / zbzgemv semantics into complex
+// binned arrays (Zb with leading dimension ldz, Yb with increment incy, in
+// complex binned elements), reproBLAS_rzgemm / rzgemv semantics in place.
+// The float overloads are the c forms.
+bool zbzgemm(int fold, char order, char ta, char tb, int64_t M, int64_t N, int64_t K, const double* alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double* Zb, int64_t ldz, hipStream_t s);
+bool zbzgemm(int fold, char order, char ta, char tb, int64_t M, int64_t N, int64_t K, const float* alpha,
+             const float* A, int64_t lda, const float* B, int64_t ldb, float* Zb, int64_t ldz, hipStream_t s);
+bool rzgemm_device(int fold, char order, char ta, char tb, int64_t M, int64_t N, int64_t K, const double* alpha,
+                   const double* A, int64_t lda, const double* B, int64_t ldb, const double* beta, double* C,
+                   int64_t ldc, hipStream_t s);
+bool rzgemm_device(int fold, char order, char ta, char tb, int64_t M, int64_t N, int64_t K, const float* alpha,
+                   const float* A, int64_t lda, const float* B, int64_t ldb, const float* beta, float* C, int64_t ldc,
+                   hipStream_t s);
+bool zbzgemv(int fold, char order, char trans, int64_t M, int64_t N, const double* alpha, const double* A,
+             int64_t lda, const double* x, int64_t incx, double* Yb, int64_t incy, hipStream_t s);
+bool zbzgemv(int fold, char order, char trans, int64_t M, int64_t N, const float* alpha, const float* A, int64_t lda,
+             const float* x, int64_t incx, float* Yb, int64_t incy, hipStream_t s);
+bool rzgemv_device(int fold, char order, char trans, int64_t M, int64_t N, const double* alpha, const double* A,
+                   int64_t lda, const double* x, int64_t incx, const double* beta, double* y, int64_t incy,
+                   hipStream_t s);
+bool rzgemv_device(int fold, char order, char trans, int64_t M, int64_t N, const float* alpha, const float* A,
+                   int64_t lda, const float* x, int64_t incx, const float* beta, float* y, int64_t incy,
                    hipStream_t s);
 
 // One gemm on device operands in the kernels' terms:

@@ -71,7 +71,10 @@ enum WorkspaceSlot : int {
   kSlotBatch = 6,
   kSlotGemmMax = 7,  // per-chunk row/column maxima of the gemm operands
   kSlotGemvX = 8,    // fl(alpha * x) gathered contiguously for the row-dot gemv
-  kSlotCount = 9,
+  kSlotZA = 9,       // complex gemm/gemv: real image of op(A)
+  kSlotZB = 10,      // complex gemm: real image of op(B); gemv: the two right-hand vectors
+  kSlotZC = 11,      // complex gemm/gemv: the outputs' real and imaginary binned values
+  kSlotCount = 12,
 };
 
 }  // namespace rb

@@ -355,9 +355,10 @@ __device__ __forceinline__ void wave_start(int fold, WaveRun<F>& w, State<CAP, F
 // Ends a wave's reduction of the tiles t0, t0 + tstep, ... of `a`: renorm,
 // exact same-index butterfly over the lanes (they share the index and scale),
 // and, if the value is poisoned, the class of the terms.
-template <int CAP, int K, int U, int MODE, class F>
-__device__ __forceinline__ void wave_finish(const Span<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                            const WaveRun<F>& w, State<CAP, F>& s) {
+// Renorm and exact same-index butterfly over the 64 lanes (they share the
+// index and scale); a poisoned value stays poisoned.
+template <int CAP, class F>
+__device__ __forceinline__ void wave_butterfly(int fold, const WaveRun<F>& w, State<CAP, F>& s) {
   renorm(fold, s.p, s.c);
   s.scl = w.scl;
 #pragma unroll
@@ -374,6 +375,12 @@ RB_UNROLL_CAP
       s.p[0] = s.p[0] + op[0];  // keep the poison visible
     }
   }
+}
+
+template <int CAP, int K, int U, int MODE, class F>
+__device__ __forceinline__ void wave_finish(const Span<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
+                                            const WaveRun<F>& w, State<CAP, F>& s) {
+  wave_butterfly(fold, w, s);
   if (nan_or_inf(uniform_value(s.p[0]))) {
     s.p[0] = wave_term_class<K, U, MODE>(a, t0, tstep, lane);
   }
