@@ -8,8 +8,10 @@ e ~ U{-40..40}), both operands resident in HBM; GB/s of algorithmic traffic
 Also reported under "extra": dsum 2^20 (uniform), dsum / dnrm2 2^28, ddot and
 dnrm2 2^28 at folds 2, 3, 4 and dsum with incX = 2, dgemv 16384^2 (N and T),
 dgemm 4096^3 (GFLOP/s and fraction of the FP64 VALU issue rate), the ragged
-batch and the MLP step, and the single-precision path (sdot / ssum / snrm2
-2^28 floats, sgemv 16384^2, sgemm 4096^3), each as one timed configuration.
+batch and the MLP step, the single-precision path (sdot / ssum / snrm2
+2^28 floats, sgemv 16384^2, sgemm 4096^3) and the complex path (zdotc / zsum /
+dznrm2 over 2^27 double complex, cdotc over 2^27 float complex, zgemv
+8192^2, zgemm 2048^3), each as one timed configuration.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -153,7 +155,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configurations)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", choices=["dsum_2p20", "dsum_2p28", "dnrm2_2p28", "folds", "dgemv", "dgemm", "ragged_batch",
-                                       "mlp", "single"],
+                                       "mlp", "single", "complex"],
                     help="run one extra configuration alone and print its record (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy and a plain torch sum of 2^28 doubles (scales the HBM counters)")
@@ -347,6 +349,8 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
         torch.cuda.empty_cache()
     if want("single"):
         ex["single"] = single(L, dev, stream, sp, vp, steps)
+    if want("complex"):
+        ex["complex"] = complex_path(L, dev, stream, sp, vp, steps)
     if want("ragged_batch"):
         ex["ragged_batch"] = ragged_batch(L, dev, stream, sp, vp)
     if want("mlp"):
@@ -408,6 +412,74 @@ def single(L, dev, stream, sp, vp, steps):
     rec["sgemm_NN_4096"] = {"ms": round(t * 1e3, 2), "GFLOP/s": round(ops / t / 1e9, 1),
                             "valu_frac": round(ops / t / VALU_PEAK_OPS_FP32, 4)}
     log(f"[bench] sgemm: {t * 1e3:.2f} ms, {ops / t / 1e9:.1f} GFLOP/s (model ops)")
+    del A, B, C
+    torch.cuda.empty_cache()
+    import reproblas_amd as rb
+
+    rb._check()
+    return rec
+
+
+def complex_path(L, dev, stream, sp, vp, steps):
+    """Complex binned reductions at fold 3: zdotc, zsum, dznrm2 over N = 2^27
+    double complex (wide-range parts), cdotc over 2^27 float complex, zgemv
+    8192^2 column-major N (beta 0.5) and zgemm 2048^3 column-major NN."""
+    import torch
+
+    rec = {}
+    n = 1 << 27
+    x = wide_gpu(2 * n, 21, dev)  # interleaved (re, im)
+    y = wide_gpu(2 * n, 22, dev)
+    out = torch.zeros(8, dtype=torch.float64, device=dev)
+    for name, fn, byts in (
+            ("zdotc", lambda: L.reproblas_hip_rzdotc_device(3, n, vp(x), 1, vp(y), 1, vp(out), sp), 32 * n),
+            ("zsum", lambda: L.reproblas_hip_rzsum_device(3, n, vp(x), 1, vp(out), sp), 16 * n),
+            ("dznrm2", lambda: L.reproblas_hip_rdznrm2_device(3, n, vp(x), 1, vp(out), sp), 16 * n)):
+        t = time_launches(fn, max(steps, 10), 10, stream)
+        rec[name + "_2p27"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
+                               "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] {name} 2^27: {t * 1e3:.3f} ms, {byts / t / 1e9:.1f} GB/s")
+    xf, yf = x.float(), y.float()
+    del x, y
+    torch.cuda.empty_cache()
+    outf = torch.zeros(8, dtype=torch.float32, device=dev)
+    t = time_launches(lambda: L.reproblas_hip_rcdotc_device(3, n, vp(xf), 1, vp(yf), 1, vp(outf), sp), max(steps, 10),
+                      20, stream)
+    rec["cdotc_2p27"] = {"ms": round(t * 1e3, 3), "GB/s": round(16 * n / t / 1e9, 1),
+                         "hbm_frac": round(16 * n / t / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"[bench] cdotc 2^27: {t * 1e3:.3f} ms")
+    del xf, yf
+    torch.cuda.empty_cache()
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    al = np.array([1.0, 0.0])
+    be = np.array([0.5, 0.0])
+    pal, pbe = ctypes.c_void_p(al.ctypes.data), ctypes.c_void_p(be.ctypes.data)
+    m = 8192
+    A = torch.randn(2 * m * m, generator=g, device=dev, dtype=torch.float64)
+    xv = torch.randn(2 * m, generator=g, device=dev, dtype=torch.float64)
+    yv = torch.randn(2 * m, generator=g, device=dev, dtype=torch.float64)
+    fn = lambda: L.reproblas_hip_rzgemv_device(3, b"C", b"N", m, m, pal, vp(A), m, vp(xv), 1, pbe, vp(yv), 1, sp)
+    t = time_launches(fn, max(steps, 10), 5, stream)
+    byts = 16 * (m * m + 3 * m)
+    rec["zgemv_N_8192"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
+                           "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"[bench] zgemv N 8192: {t * 1e3:.3f} ms")
+    del A
+    torch.cuda.empty_cache()
+    k = 2048
+    A = torch.randn(2 * k * k, generator=g, device=dev, dtype=torch.float64)
+    B = torch.randn(2 * k * k, generator=g, device=dev, dtype=torch.float64)
+    C = torch.zeros(2 * k * k, device=dev, dtype=torch.float64)
+    bz = np.array([0.0, 0.0])
+    pbz = ctypes.c_void_p(bz.ctypes.data)
+    fn = lambda: L.reproblas_hip_rzgemm_device(3, b"C", b"N", b"N", k, k, k, pal, vp(A), k, vp(B), k, pbz, vp(C), k,
+                                               sp)
+    t = time_launches(fn, 3, 2, stream)
+    ops = 4 * 11.0 * k ** 3  # four real products per complex product
+    rec["zgemm_NN_2048"] = {"ms": round(t * 1e3, 2), "GFLOP/s": round(ops / t / 1e9, 1),
+                            "valu_frac": round(ops / t / VALU_PEAK_OPS, 4)}
+    log(f"[bench] zgemm 2048: {t * 1e3:.2f} ms")
     del A, B, C
     torch.cuda.empty_cache()
     import reproblas_amd as rb

@@ -15,10 +15,11 @@
 // the complex product is one real binned gemm of M x 2K by 2K x 2N whose
 // outputs are the real (columns 0..N-1) and imaginary (N..2N-1) parts, each
 // a binned sum over exactly the reference's terms: the real kernels' bits
-// equal the reference's. gemv runs the real row-dot gemv twice over the
-// rows of op(A) with the right-hand vectors (x_r, -x_i) and (x_i, x_r)
-// interleaved (conjugated rows: (x_r, x_i) and (x_i, -x_r)); when op(A)'s
-// rows are not contiguous in memory they are first copied out.
+// equal the reference's. gemv with contiguous rows of op(A) runs the real
+// row-dot gemv twice over the interleaved rows with the right-hand vectors
+// (x_r, -x_i) and (x_i, x_r) (conjugated rows: (x_r, x_i) and (x_i, -x_r));
+// with terms lda apart it runs the complex mode of the column gemv
+// (zgemv_cols), one pass over A.
 #include <algorithm>
 
 #include "binned_core.hpp"
@@ -227,17 +228,14 @@ __global__ __launch_bounds__(kT) void gemv_vectors_kernel(int64_t n, const F* x,
   }
 }
 
-// Rows of op(A) copied out contiguously (row i: op(A)_i. interleaved).
+// alpha*x gathered contiguously (interleaved) for the column path.
 template <class F>
-__global__ __launch_bounds__(kT) void gemv_rows_kernel(int64_t Mo, int64_t No, int rm, int trans, const F* A,
-                                                       int64_t lda, F* RA) {
-  const int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (e >= Mo * No) return;
-  const int64_t i = e / No, j = e % No;
-  const int64_t r = trans == 0 ? i : j, c = trans == 0 ? j : i;
-  const int64_t off = rm ? r * lda + c : c * lda + r;
-  RA[2 * (i * No + j)] = A[2 * off];
-  RA[2 * (i * No + j) + 1] = A[2 * off + 1];
+__global__ __launch_bounds__(kT) void alpha_x_kernel(int64_t n, const F* x, int64_t incx, F al_r, F al_i, int mode,
+                                                     F* xs) {
+  const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (j >= n) return;
+  const F* p = x + 2 * j * incx;
+  times_scalar(mode, p[0], p[1], al_r, al_i, xs[2 * j], xs[2 * j + 1]);
 }
 
 inline int trans_code(char t) { return is_n(t) ? 0 : (is_t(t) ? 1 : 2); }
@@ -255,16 +253,17 @@ bool real_gemm<float>(int fold, int64_t M, int64_t N, int64_t K, const float* A,
   return sbsgemm(fold, 'R', 'N', 'N', M, N, K, 1.0f, A, K, B, N, Cb, N, s);
 }
 template <class F>
-bool real_gemv(int fold, int64_t M, int64_t N, const F* A, int64_t lda, const F* x, F* Yb, hipStream_t s);
+bool real_gemv(int fold, int64_t M, int64_t N, const F* A, int64_t lda, const F* x, F* Yb, int64_t incYb,
+               hipStream_t s);
 template <>
 bool real_gemv<double>(int fold, int64_t M, int64_t N, const double* A, int64_t lda, const double* x, double* Yb,
-                       hipStream_t s) {
-  return dbdgemv(fold, 'R', 'N', M, N, 1.0, A, lda, x, 1, Yb, 1, s);
+                       int64_t incYb, hipStream_t s) {
+  return dbdgemv(fold, 'R', 'N', M, N, 1.0, A, lda, x, 1, Yb, incYb, s);
 }
 template <>
 bool real_gemv<float>(int fold, int64_t M, int64_t N, const float* A, int64_t lda, const float* x, float* Yb,
-                      hipStream_t s) {
-  return sbsgemv(fold, 'R', 'N', M, N, 1.0f, A, lda, x, 1, Yb, 1, s);
+                      int64_t incYb, hipStream_t s) {
+  return sbsgemv(fold, 'R', 'N', M, N, 1.0f, A, lda, x, 1, Yb, incYb, s);
 }
 
 template <class F>
@@ -307,37 +306,33 @@ bool zgemm_into(int fold, char order, char ta, char tb, int64_t M, int64_t N, in
   return real_gemm<F>(fold, M, 2 * N, 2 * K, RA, RB, R, s);
 }
 
+// Accumulates the real and imaginary parts of output i into R[2i], R[2i+1].
 template <class F>
 bool zgemv_into(int fold, char order, char trans, int64_t M, int64_t N, const F* alpha, const F* A, int64_t lda,
-                const F* x, int64_t incx, F* R0, F* R1, hipStream_t s) {
+                const F* x, int64_t incx, F* R, hipStream_t s) {
   if (M <= 0 || N <= 0 || (alpha[0] == F(0) && alpha[1] == F(0))) return true;
   const bool rm = row_major(order);
   const int tc = trans_code(trans);
   const int64_t Mo = tc == 0 ? M : N, No = tc == 0 ? N : M;
   const int mode = (alpha[0] == F(1) && alpha[1] == F(0)) ? 0 : 1;
-  // rows of op(A) are contiguous for row-major 'N' and column-major 'T'/'C'
+  const int conj = (tc == 2) ? 1 : 0;
+  // rows of op(A) are contiguous for row-major 'N' and column-major 'T'/'C':
+  // two real row-dot passes over the interleaved rows; otherwise the terms of
+  // an output are lda apart: one complex column pass
   const bool rows_contig = rm ? (tc == 0) : (tc != 0);
-  const F* RA;
-  int64_t ldr;
-  bool conj;
-  if (rows_contig) {
-    RA = A;
-    ldr = 2 * lda;
-    conj = (tc == 2);
-  } else {
-    F* T = (F*)workspace(s, kSlotZA, sizeof(F) * (size_t)(Mo * 2 * No));
-    if (!T) return false;
-    // copied as stored; a conjugation is folded into the vectors either way
-    gemv_rows_kernel<F><<<grid1(Mo * No), kT, 0, s>>>(Mo, No, rm, tc, A, lda, T);
-    RA = T;
-    ldr = 2 * No;
-    conj = (tc == 2);
+  if (!rows_contig) {
+    F* xs = (F*)workspace(s, kSlotZB, sizeof(F) * (size_t)(2 * No));
+    if (!xs) return false;
+    alpha_x_kernel<F><<<grid1(No), kT, 0, s>>>(No, x, incx, alpha[0], alpha[1], mode, xs);
+    if (!RB_HIP(hipGetLastError())) return false;
+    return zgemv_cols(fold, Mo, No, A, lda, conj, xs, R, s);
   }
   F* V = (F*)workspace(s, kSlotZB, sizeof(F) * (size_t)(4 * No));
   if (!V) return false;
-  gemv_vectors_kernel<F><<<grid1(No), kT, 0, s>>>(No, x, incx, alpha[0], alpha[1], mode, conj ? 1 : 0, V, V + 2 * No);
+  gemv_vectors_kernel<F><<<grid1(No), kT, 0, s>>>(No, x, incx, alpha[0], alpha[1], mode, conj, V, V + 2 * No);
   if (!RB_HIP(hipGetLastError())) return false;
-  return real_gemv<F>(fold, Mo, 2 * No, RA, ldr, V, R0, s) && real_gemv<F>(fold, Mo, 2 * No, RA, ldr, V + 2 * No, R1, s);
+  return real_gemv<F>(fold, Mo, 2 * No, A, 2 * lda, V, R, 2, s) &&
+         real_gemv<F>(fold, Mo, 2 * No, A, 2 * lda, V + 2 * No, R + 2 * fold, 2, s);
 }
 
 template <class F>
@@ -377,10 +372,11 @@ bool zbzgemv_impl(int fold, char order, char trans, int64_t M, int64_t N, const 
   const int64_t Mo = is_n(trans) ? M : N;
   F* R = (F*)workspace(s, kSlotZC, sizeof(F) * (size_t)(2 * Mo * 2 * fold));
   if (!R) return false;
-  // Y as an Mo x 1 column with leading dimension incy (row-major view)
-  split_kernel<F><<<grid1(Mo), kT, 0, s>>>(fold, Mo, 1, Yb, incy, 1, R, 1, Mo);
-  if (!zgemv_into<F>(fold, order, trans, M, N, alpha, A, lda, x, incx, R, R + Mo * 2 * fold, s)) return false;
-  join_kernel<F><<<grid1(Mo), kT, 0, s>>>(fold, Mo, 1, R, 1, Mo, Yb, incy, 1);
+  // Y as an Mo x 1 column with leading dimension incy (row-major view);
+  // parts interleaved in R
+  split_kernel<F><<<grid1(Mo), kT, 0, s>>>(fold, Mo, 1, Yb, incy, 1, R, 2, 1);
+  if (!zgemv_into<F>(fold, order, trans, M, N, alpha, A, lda, x, incx, R, s)) return false;
+  join_kernel<F><<<grid1(Mo), kT, 0, s>>>(fold, Mo, 1, R, 2, 1, Yb, incy, 1);
   return RB_HIP(hipGetLastError());
 }
 
@@ -393,9 +389,9 @@ bool rzgemv_impl(int fold, char order, char trans, int64_t M, int64_t N, const F
   F* R = (F*)workspace(s, kSlotZC, sizeof(F) * (size_t)(2 * Mo * 2 * fold));
   if (!R) return false;
   const int bmode = (beta[0] == F(0) && beta[1] == F(0)) ? 0 : ((beta[0] == F(1) && beta[1] == F(0)) ? 1 : 2);
-  RB_ZFOLD_LAUNCH(beta_split_kernel, F, grid1(Mo), fold, Mo, 1, y, incy, 1, beta[0], beta[1], bmode, R, 1, Mo);
-  if (!zgemv_into<F>(fold, order, trans, M, N, alpha, A, lda, x, incx, R, R + Mo * 2 * fold, s)) return false;
-  RB_ZFOLD_LAUNCH(conv_join_kernel, F, grid1(Mo), fold, Mo, 1, R, 1, Mo, y, incy, 1);
+  RB_ZFOLD_LAUNCH(beta_split_kernel, F, grid1(Mo), fold, Mo, 1, y, incy, 1, beta[0], beta[1], bmode, R, 2, 1);
+  if (!zgemv_into<F>(fold, order, trans, M, N, alpha, A, lda, x, incx, R, s)) return false;
+  RB_ZFOLD_LAUNCH(conv_join_kernel, F, grid1(Mo), fold, Mo, 1, R, 2, 1, y, incy, 1);
   return RB_HIP(hipGetLastError());
 }
 

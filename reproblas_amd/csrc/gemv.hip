@@ -50,6 +50,7 @@ struct GemvArgs {
   int64_t chunk;     // col-axpy: columns per chunk
   int64_t chunks;
   int fold;
+  int conj;  // complex col-axpy: conjugate A's elements
 };
 
 template <class F>
@@ -171,9 +172,26 @@ __global__ __launch_bounds__(kThreads) void rowdot_kernel(GemvArgs<F> a, const F
 // --------------------------------------------------------------- col-axpy --
 // Block: 128 outputs (2 per lane) x one column chunk; wave w takes columns
 // chunk_start + w, +4, +8, ...
-template <int CAP, bool FIXED, bool VEC, class F>
+//
+// CPLX: the two outputs of a lane are the real and imaginary parts of one
+// complex output i (o0 = 2i): A holds interleaved complex elements (lda in
+// complex elements), x the contiguous interleaved alpha*x, and each column
+// k gives every part two terms, a_r*x_r and -(a_i*x_i) (conj: a_i*x_i) for
+// the real part, a_i*x_r (conj: -(a_i*x_r)) and a_r*x_i for the imaginary
+// one -- the terms of the reference's zbzdotu / zbzdotc (zmzdotu.ccog,
+// zmzdotc.ccog).
+template <class F>
+__device__ __forceinline__ void complex_terms(F ar, F ai, F xr, F xi, int conj, F& r0, F& r1, F& i0, F& i1) {
+  r0 = ar * xr;
+  r1 = conj ? ai * xi : (ai * xi) * F(-1);
+  i0 = conj ? (ai * xr) * F(-1) : ai * xr;
+  i1 = ar * xi;
+}
+
+template <int CAP, bool FIXED, bool VEC, class F, bool CPLX = false>
 __global__ __launch_bounds__(kThreads) void colaxpy_kernel(GemvArgs<F> a) {
   constexpr int U = 4;  // columns per step
+  constexpr int TU = CPLX ? 2 * U : U;  // terms per output per step
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar)
@@ -190,13 +208,39 @@ __global__ __launch_bounds__(kThreads) void colaxpy_kernel(GemvArgs<F> a) {
   F lim0 = admit_limit<F>(Fmt<F>::max_index), lim1 = lim0;
   int dep = 0;
   for (int64_t k = kbeg + wave; k < kend; k += kWaves * U) {
-    F t0[U], t1[U];
+    F t0[TU], t1[TU];
     F m0 = F(0), m1 = F(0);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t kk = k + (int64_t)u * kWaves;
       const bool kin = kk < kend;
       F a0 = F(0), a1 = F(0), xs = F(0);
+      if constexpr (CPLX) {
+        F xr = F(0), xi = F(0);
+        if (kin && in0) {
+          const F* el = a.A + 2 * (kk * a.lda + (o0 >> 1));
+          if constexpr (!VEC) {
+            a0 = el[0];
+            a1 = el[1];
+          } else if constexpr (sizeof(F) == 8) {
+            const double2 d = *reinterpret_cast<const double2*>(el);
+            a0 = d.x;
+            a1 = d.y;
+          } else {
+            const float2 d = *reinterpret_cast<const float2*>(el);
+            a0 = d.x;
+            a1 = d.y;
+          }
+        }
+        if (kin) {
+          xr = a.x[2 * kk];
+          xi = a.x[2 * kk + 1];
+        }
+        complex_terms(a0, a1, xr, xi, a.conj, t0[u], t0[U + u], t1[u], t1[U + u]);
+        m0 = fmax(m0, fmax(fabs(t0[u]), fabs(t0[U + u])));
+        m1 = fmax(m1, fmax(fabs(t1[u]), fabs(t1[U + u])));
+        continue;
+      }
       if (kin) {
         xs = scaled_x(a, kk);
         const F* col = a.A + kk * a.lda + o0;
@@ -220,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void colaxpy_kernel(GemvArgs<F> a) {
       m0 = fmax(m0, fabs(t0[u]));
       m1 = fmax(m1, fabs(t1[u]));
     }
-    if (dep + U > Fmt<F>::endurance) {
+    if (dep + TU > Fmt<F>::endurance) {
       renorm(fold, p0, c0);
       renorm(fold, p1, c1);
       dep = 0;
@@ -241,11 +285,11 @@ __global__ __launch_bounds__(kThreads) void colaxpy_kernel(GemvArgs<F> a) {
     }
     const bool top0 = idx0 == 0, top1 = idx1 == 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < TU; ++u) {
       deposit(fold, top0, t0[u], p0);
       deposit(fold, top1, t1[u], p1);
     }
-    dep += U;
+    dep += TU;
   }
   renorm(fold, p0, c0);
   renorm(fold, p1, c1);
@@ -295,7 +339,7 @@ RB_UNROLL_CAP
 }
 
 // merge the per-chunk partials of each output, fold in beta*y (or Yb), convert
-template <int CAP, bool FIXED, class F>
+template <int CAP, bool FIXED, class F, bool CPLX = false>
 __global__ __launch_bounds__(kThreads) void colaxpy_finish(GemvArgs<F> a) {
   const int fold = FIXED ? CAP : a.fold;
   const int64_t o = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -317,10 +361,21 @@ RB_UNROLL_CAP
   if (nan_or_inf(p[0])) {
     bool nan = false, pinf = false, ninf = false;
     for (int64_t k = 0; k < a.len; ++k) {
-      const F t = a.A[o + k * a.lda] * scaled_x(a, k);
-      nan |= (t != t);
-      pinf |= (t == (F)__builtin_inf());
-      ninf |= (t == -(F)__builtin_inf());
+      F t[2];
+      if constexpr (CPLX) {
+        const F* el = a.A + 2 * (k * a.lda + (o >> 1));
+        F r0, r1, i0, i1;
+        complex_terms(el[0], el[1], a.x[2 * k], a.x[2 * k + 1], a.conj, r0, r1, i0, i1);
+        t[0] = (o & 1) ? i0 : r0;
+        t[1] = (o & 1) ? i1 : r1;
+      } else {
+        t[0] = t[1] = a.A[o + k * a.lda] * scaled_x(a, k);
+      }
+      for (int q = 0; q < 2; ++q) {
+        nan |= (t[q] != t[q]);
+        pinf |= (t[q] == (F)__builtin_inf());
+        ninf |= (t[q] == -(F)__builtin_inf());
+      }
     }
     p[0] = (nan || (pinf && ninf) || !(pinf || ninf)) ? (F)__builtin_nan("")
                                                       : (pinf ? (F)__builtin_inf() : -(F)__builtin_inf());
@@ -334,6 +389,9 @@ RB_UNROLL_CAP
 #define RB_GEMV_ROW_BLOCKS_PER_CU 4
 #endif
 constexpr int kRowBlocksPerCU = RB_GEMV_ROW_BLOCKS_PER_CU;
+
+template <int CAP, bool FIXED, class F, bool CPLX>
+bool launch_cols(GemvArgs<F>& a, hipStream_t s);
 
 template <int CAP, bool FIXED, class F>
 bool launch_gemv(bool rowdot, GemvArgs<F>& a, hipStream_t s) {
@@ -351,6 +409,15 @@ bool launch_gemv(bool rowdot, GemvArgs<F>& a, hipStream_t s) {
       rowdot_kernel<CAP, FIXED, 1, F><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
     return RB_HIP(hipGetLastError());
   }
+  return launch_cols<CAP, FIXED, F, false>(a, s);
+}
+
+// col-axpy launch: column chunks so that about four workgroups per CU run,
+// then the merge of the per-chunk partials
+template <int CAP, bool FIXED, class F, bool CPLX>
+bool launch_cols(GemvArgs<F>& a, hipStream_t s) {
+  const DeviceInfo* d = device_info();
+  if (!d) return false;
   const int64_t rowblocks = (a.nout + 127) / 128;
   int64_t chunks = (4 * (int64_t)d->cus + rowblocks - 1) / rowblocks;
   const int64_t max_chunks = (a.len + 255) / 256;  // keep >= 256 columns per chunk
@@ -362,14 +429,15 @@ bool launch_gemv(bool rowdot, GemvArgs<F>& a, hipStream_t s) {
   const int64_t records = (CAP <= 8) ? a.chunks : a.chunks * kWaves;
   a.partials = (F*)workspace(s, kSlotPartials, sizeof(F) * (size_t)records * a.nout * 2 * a.fold);
   if (!a.partials) return false;
-  const bool vec = ((uintptr_t)a.A % (2 * sizeof(F)) == 0) && (a.lda % 2 == 0);
+  // vector loads: two adjacent real elements, or one complex element, aligned
+  const bool vec = ((uintptr_t)a.A % (2 * sizeof(F)) == 0) && (CPLX || a.lda % 2 == 0);
   dim3 grid((unsigned)rowblocks, (unsigned)a.chunks);
   if (vec)
-    colaxpy_kernel<CAP, FIXED, true, F><<<grid, kThreads, 0, s>>>(a);
+    colaxpy_kernel<CAP, FIXED, true, F, CPLX><<<grid, kThreads, 0, s>>>(a);
   else
-    colaxpy_kernel<CAP, FIXED, false, F><<<grid, kThreads, 0, s>>>(a);
+    colaxpy_kernel<CAP, FIXED, false, F, CPLX><<<grid, kThreads, 0, s>>>(a);
   if (!RB_HIP(hipGetLastError())) return false;
-  colaxpy_finish<CAP, FIXED, F><<<(unsigned)((a.nout + kThreads - 1) / kThreads), kThreads, 0, s>>>(a);
+  colaxpy_finish<CAP, FIXED, F, CPLX><<<(unsigned)((a.nout + kThreads - 1) / kThreads), kThreads, 0, s>>>(a);
   return RB_HIP(hipGetLastError());
 }
 
@@ -398,6 +466,7 @@ bool gemv_common(int fold, char order, char trans, int64_t M, int64_t N, F alpha
   a.partials = nullptr;
   a.chunk = a.chunks = 0;
   a.fold = fold;
+  a.conj = 0;
   if (a.nout <= 0) return true;
   // the terms of an output are contiguous for row-major N and column-major T
   const bool rowdot = (rowmajor == notrans);
@@ -408,6 +477,41 @@ bool gemv_common(int fold, char order, char trans, int64_t M, int64_t N, F alpha
     default:
       if (fold <= 8) return launch_gemv<8, false>(rowdot, a, s);
       return launch_gemv<Fmt<F>::max_fold, false>(rowdot, a, s);
+  }
+}
+
+// Complex outputs whose terms are lda apart (column-major N, row-major T/C):
+// R[2i] and R[2i+1] (real binned) accumulate the real and imaginary parts of
+// output i over sum_k A[k*lda + i] * xs[k] (complex, A conjugated if conj);
+// xs: alpha*x, contiguous interleaved.
+template <class F>
+bool zcols_impl(int fold, int64_t Mo, int64_t No, const F* A, int64_t lda, int conj, const F* xs, F* R,
+                hipStream_t s) {
+  GemvArgs<F> a;
+  a.nout = 2 * Mo;
+  a.len = No;
+  a.A = A;
+  a.lda = lda;
+  a.x = xs;
+  a.incx = 1;
+  a.alpha = F(1);
+  a.Yb = R;
+  a.incYb = 1;
+  a.y = nullptr;
+  a.incy = 0;
+  a.beta = F(0);
+  a.partials = nullptr;
+  a.chunk = a.chunks = 0;
+  a.fold = fold;
+  a.conj = conj;
+  if (Mo <= 0 || No <= 0) return true;
+  switch (fold) {
+    case 2: return launch_cols<2, true, F, true>(a, s);
+    case 3: return launch_cols<3, true, F, true>(a, s);
+    case 4: return launch_cols<4, true, F, true>(a, s);
+    default:
+      if (fold <= 8) return launch_cols<8, false, F, true>(a, s);
+      return launch_cols<Fmt<F>::max_fold, false, F, true>(a, s);
   }
 }
 
@@ -428,6 +532,15 @@ bool rgemv_impl(int fold, char order, char trans, int64_t M, int64_t N, F alpha,
 }
 
 }  // namespace
+
+bool zgemv_cols(int fold, int64_t Mo, int64_t No, const double* A, int64_t lda, int conj, const double* xs,
+                double* R, hipStream_t s) {
+  return zcols_impl<double>(fold, Mo, No, A, lda, conj, xs, R, s);
+}
+bool zgemv_cols(int fold, int64_t Mo, int64_t No, const float* A, int64_t lda, int conj, const float* xs, float* R,
+                hipStream_t s) {
+  return zcols_impl<float>(fold, Mo, No, A, lda, conj, xs, R, s);
+}
 
 bool dbdgemv(int fold, char order, char trans, int64_t M, int64_t N, double alpha, const double* A, int64_t lda,
              const double* x, int64_t incx, double* Yb, int64_t incYb, hipStream_t s) {

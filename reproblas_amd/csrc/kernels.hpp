@@ -103,6 +103,15 @@ bool rdgemm_device(int fold, char order, char transa, char transb, int64_t M, in
                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
                    hipStream_t s);
 
+// Complex gemv outputs whose terms are lda apart (column-major N, row-major
+// T/C) in one pass: R[2i], R[2i+1] (real binned, accumulated into) get the
+// real and imaginary parts of sum_k op(A)_ik * xs[k]; A interleaved complex
+// with op(A)_ik = A[k*lda + i] (conjugated when conj), xs = alpha*x contiguous.
+bool zgemv_cols(int fold, int64_t Mo, int64_t No, const double* A, int64_t lda, int conj, const double* xs,
+                double* R, hipStream_t s);
+bool zgemv_cols(int fold, int64_t Mo, int64_t No, const float* A, int64_t lda, int conj, const float* xs, float* R,
+                hipStream_t s);
+
 // Complex gemm / gemv on device operands (interleaved pairs; alpha and beta
 // are host pairs): binnedBLAS_zbzgemm / zbzgemv semantics into complex
 // binned arrays (Zb with leading dimension ldz, Yb with increment incy, in

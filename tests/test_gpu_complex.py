@@ -411,3 +411,33 @@ def test_device_gemm_gemv(gpu, prec):
     assert getattr(L, f"reproblas_hip_r{prec}gemv_device")(3, b"C", b"N", M, K, O.p(al), vp(tA), M, vp(tx), 1,
                                                            O.p(be), vp(ty), 1, s) == 0
     assert bits_eq(ty.cpu().numpy(), ye)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_gemv_columns_unaligned_matrix(gpu, prec):
+    """Column-layout complex gemv with A at an address aligned only to its
+    real type (element loads instead of vector loads): same bits."""
+    import torch
+
+    import reproblas_amd as rb
+
+    O = oracle_z(prec)
+    M, N = 257, 300
+    A = cvec(prec, "wide", M * N, 96)
+    x = cvec(prec, "wide", N, 97)
+    y = cvec(prec, "uniform", M, 98)
+    ye = y.copy()
+    al, be = O.scalar(complex(1.25, -0.5)), O.scalar(complex(0.5, 0))
+    O.fn["rgemv"](3, b"C", b"N", M, N, O.p(al), O.p(iv(A)), M, O.p(iv(x)), 1, O.p(be), O.p(iv(ye)), 1)
+    tdt = torch.float64 if prec == "z" else torch.float32
+    buf = torch.zeros(2 * M * N + 1, dtype=tdt, device=gpu)
+    buf[1:] = torch.from_numpy(iv(A).copy()).to(gpu)
+    tx = torch.from_numpy(iv(x).copy()).to(gpu)
+    ty = torch.from_numpy(iv(y).copy()).to(gpu)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = rb.lib()
+    pa = ctypes.c_void_p(buf.data_ptr() + buf.element_size())
+    assert getattr(L, f"reproblas_hip_r{prec}gemv_device")(3, b"C", b"N", M, N, O.p(al), pa, M,
+                                                           ctypes.c_void_p(tx.data_ptr()), 1, O.p(be),
+                                                           ctypes.c_void_p(ty.data_ptr()), 1, s) == 0
+    assert bits_eq(ty.cpu().numpy(), iv(ye))
