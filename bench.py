@@ -11,7 +11,7 @@ dgemm 4096^3 (GFLOP/s and fraction of the FP64 VALU issue rate), the ragged
 batch and the MLP step, the single-precision path (sdot / ssum / snrm2
 2^28 floats, sgemv 16384^2, sgemm 4096^3) and the complex path (zdotc / zsum /
 dznrm2 over 2^27 double complex, cdotc over 2^27 float complex, zgemv
-8192^2, zgemm 2048^3), each as one timed configuration.
+8192^2 N and T, zgemm 2048^3), each as one timed configuration.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -459,12 +459,14 @@ def complex_path(L, dev, stream, sp, vp, steps):
     A = torch.randn(2 * m * m, generator=g, device=dev, dtype=torch.float64)
     xv = torch.randn(2 * m, generator=g, device=dev, dtype=torch.float64)
     yv = torch.randn(2 * m, generator=g, device=dev, dtype=torch.float64)
-    fn = lambda: L.reproblas_hip_rzgemv_device(3, b"C", b"N", m, m, pal, vp(A), m, vp(xv), 1, pbe, vp(yv), 1, sp)
-    t = time_launches(fn, max(steps, 10), 5, stream)
-    byts = 16 * (m * m + 3 * m)
-    rec["zgemv_N_8192"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
-                           "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
-    log(f"[bench] zgemv N 8192: {t * 1e3:.3f} ms")
+    for tr in ("N", "T"):
+        fn = lambda: L.reproblas_hip_rzgemv_device(3, b"C", tr.encode(), m, m, pal, vp(A), m, vp(xv), 1, pbe, vp(yv),
+                                                   1, sp)
+        t = time_launches(fn, max(steps, 10), 5, stream)
+        byts = 16 * (m * m + 3 * m)
+        rec[f"zgemv_{tr}_8192"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
+                                   "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] zgemv {tr} 8192: {t * 1e3:.3f} ms")
     del A
     torch.cuda.empty_cache()
     k = 2048

@@ -407,6 +407,156 @@ constexpr int z_units() {
   return (is_dot(T) ? 64 : 128) / (2 * (int)sizeof(F));
 }
 
+// ------------------------------------------------------ complex row gemv --
+// Rows of op(A) contiguous (row-major N, column-major T/C): output o is the
+// complex dotu (conj: dotc) of row o, A + 2*o*lda, with xs = alpha*x. A
+// persistent grid of waves owns rows o, o + W, ... and walks their tiles as
+// one sequence (the next tile -- at a row's end, the first of the next row --
+// in flight while the current one is deposited), as the real row-dot gemv.
+// Output o accumulates into the real binned values R[2o] (real part) and
+// R[2o+1] (imaginary part).
+template <class F>
+struct ZRowArgs {
+  int64_t nout, len;
+  const F* A;
+  int64_t lda;  // complex elements
+  const F* xs;  // alpha*x, contiguous interleaved
+  F* R;
+  int fold;
+};
+
+template <int U, class F>
+__device__ __forceinline__ ZSpan<F> zrow_span(const ZRowArgs<F>& a, int64_t o) {
+  return make_zspan<U>(a.A + 2 * o * a.lda, 1, a.xs, 1, a.len);
+}
+
+template <int CAP, int T, int U, int MODE, class F>
+__device__ __forceinline__ void zfinish_row(const ZRowArgs<F>& a, int fold, int64_t o, int lane,
+                                            WaveRun<F> (&w)[2], State<CAP, F> (&s)[2]) {
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    wave_butterfly(fold, w[ch], s[ch]);
+    if (nan_or_inf(uniform_value(s[ch].p[0])))
+      s[ch].p[0] = zwave_term_class<T, U, MODE>(zrow_span<U>(a, o), 0, 1, lane, ch);
+    if (lane == 0) {
+      F* r = a.R + (2 * o + ch) * 2 * fold;
+      F py[CAP], cy[CAP];
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          py[j] = r[j];
+          cy[j] = r[fold + j];
+        }
+      renorm(fold, py, cy);
+      add(fold, py, cy, s[ch].p, s[ch].c);
+      renorm(fold, py, cy);
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          r[j] = py[j];
+          r[fold + j] = cy[j];
+        }
+    }
+  }
+}
+
+template <int CAP, bool FIXED, bool CONJ, int MODE, class F>
+__global__ __launch_bounds__(kThreads) void zrowdot_kernel(ZRowArgs<F> a) {
+  constexpr int T = CONJ ? kZDotC : kZDotU;
+  constexpr int U = z_units<T, F>();
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * kWaves;
+  int64_t o = (int64_t)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (o >= a.nout) return;  // whole wave leaves together
+  const int64_t tiles = (a.len + 64 * U - 1) / (64 * U);
+  State<CAP, F> s[2];
+  WaveRun<F> w[2];
+  wave_start(fold, w[0], s[0]);
+  wave_start(fold, w[1], s[1]);
+  F axr[U], axi[U], ayr[U], ayi[U], bxr[U], bxi[U], byr[U], byi[U];
+  int64_t t = 0;
+  load_ztile<T, U, MODE>(zrow_span<U>(a, o), t, lane, axr, axi, ayr, ayi);
+  while (true) {
+    int64_t on = o, tn = t + 1;
+    if (tn == tiles) {
+      on = o + nw;
+      tn = 0;
+    }
+    if (on < a.nout) load_ztile<T, U, MODE>(zrow_span<U>(a, on), tn, lane, bxr, bxi, byr, byi);
+    deposit_ztile<CAP, T, U>(fold, axr, axi, ayr, ayi, w, s);
+    if (tn == 0) {
+      zfinish_row<CAP, T, U, MODE>(a, fold, o, lane, w, s);
+      wave_start(fold, w[0], s[0]);
+      wave_start(fold, w[1], s[1]);
+    }
+    if (on >= a.nout) break;
+    o = on;
+    t = tn;
+    on = o;
+    tn = t + 1;
+    if (tn == tiles) {
+      on = o + nw;
+      tn = 0;
+    }
+    if (on < a.nout) load_ztile<T, U, MODE>(zrow_span<U>(a, on), tn, lane, axr, axi, ayr, ayi);
+    deposit_ztile<CAP, T, U>(fold, bxr, bxi, byr, byi, w, s);
+    if (tn == 0) {
+      zfinish_row<CAP, T, U, MODE>(a, fold, o, lane, w, s);
+      wave_start(fold, w[0], s[0]);
+      wave_start(fold, w[1], s[1]);
+    }
+    if (on >= a.nout) break;
+    o = on;
+    t = tn;
+  }
+}
+
+template <int CAP, bool FIXED, class F>
+bool zlaunch_rows(const ZRowArgs<F>& a, bool conj, int mode, int blocks, hipStream_t s) {
+  if (conj) {
+    if (mode == 0)
+      zrowdot_kernel<CAP, FIXED, true, 0, F><<<blocks, kThreads, 0, s>>>(a);
+    else
+      zrowdot_kernel<CAP, FIXED, true, 1, F><<<blocks, kThreads, 0, s>>>(a);
+  } else {
+    if (mode == 0)
+      zrowdot_kernel<CAP, FIXED, false, 0, F><<<blocks, kThreads, 0, s>>>(a);
+    else
+      zrowdot_kernel<CAP, FIXED, false, 1, F><<<blocks, kThreads, 0, s>>>(a);
+  }
+  return RB_HIP(hipGetLastError());
+}
+
+template <class F>
+bool zrows_impl(int fold, int64_t Mo, int64_t No, const F* A, int64_t lda, int conj, const F* xs, F* R,
+                hipStream_t s) {
+  if (Mo <= 0 || No <= 0) return true;
+  const DeviceInfo* dev = device_info();
+  if (!dev) return false;
+  ZRowArgs<F> a;
+  a.nout = Mo;
+  a.len = No;
+  a.A = A;
+  a.lda = lda;
+  a.xs = xs;
+  a.R = R;
+  a.fold = fold;
+  // 16-byte loads need every row start 16-byte aligned
+  const bool aligned = ((uintptr_t)A % 16 == 0) && ((uintptr_t)xs % 16 == 0) &&
+                       ((2 * sizeof(F) * (size_t)lda) % 16 == 0);
+  const int mode = aligned ? 0 : 1;
+  const int64_t blocks = std::min<int64_t>((Mo + kWaves - 1) / kWaves, (int64_t)dev->cus * 4);
+  switch (fold) {
+    case 2: return zlaunch_rows<2, true>(a, conj, mode, (int)blocks, s);
+    case 3: return zlaunch_rows<3, true>(a, conj, mode, (int)blocks, s);
+    case 4: return zlaunch_rows<4, true>(a, conj, mode, (int)blocks, s);
+    default:
+      if (fold <= 8) return zlaunch_rows<8, false>(a, conj, mode, (int)blocks, s);
+      return zlaunch_rows<Fmt<F>::max_fold, false>(a, conj, mode, (int)blocks, s);
+  }
+}
+
 template <int CAP, bool FIXED, int T, class F>
 bool zlaunch_mode(int mode, const ZArgs<F>& a, int blocks, hipStream_t s) {
   constexpr int U = z_units<T, F>();
@@ -502,6 +652,15 @@ bool zblas1(int fold, ZTerm term, int64_t n, const float* x, int64_t incx, const
             float* result, hipStream_t s) {
   return zblas1_impl<float>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out,
                             scale_out, result, s);
+}
+
+bool zgemv_rows(int fold, int64_t Mo, int64_t No, const double* A, int64_t lda, int conj, const double* xs,
+                double* R, hipStream_t s) {
+  return zrows_impl<double>(fold, Mo, No, A, lda, conj, xs, R, s);
+}
+bool zgemv_rows(int fold, int64_t Mo, int64_t No, const float* A, int64_t lda, int conj, const float* xs, float* R,
+                hipStream_t s) {
+  return zrows_impl<float>(fold, Mo, No, A, lda, conj, xs, R, s);
 }
 
 }  // namespace rb

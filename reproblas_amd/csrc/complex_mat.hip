@@ -15,11 +15,9 @@
 // the complex product is one real binned gemm of M x 2K by 2K x 2N whose
 // outputs are the real (columns 0..N-1) and imaginary (N..2N-1) parts, each
 // a binned sum over exactly the reference's terms: the real kernels' bits
-// equal the reference's. gemv with contiguous rows of op(A) runs the real
-// row-dot gemv twice over the interleaved rows with the right-hand vectors
-// (x_r, -x_i) and (x_i, x_r) (conjugated rows: (x_r, x_i) and (x_i, -x_r));
-// with terms lda apart it runs the complex mode of the column gemv
-// (zgemv_cols), one pass over A.
+// equal the reference's. gemv reads A once: with contiguous rows of op(A) a
+// complex row-dot kernel (complex.hip, zgemv_rows), with terms lda apart the
+// complex mode of the column gemv (gemv.hip, zgemv_cols).
 #include <algorithm>
 
 #include "binned_core.hpp"
@@ -204,31 +202,7 @@ RB_UNROLL_CAP
   }
 }
 
-// The two right-hand vectors of the gemv (length 2n, interleaved) from
-// x' = alpha*x: plain rows (x'_r, -x'_i), (x'_i, x'_r); conjugated rows
-// (x'_r, x'_i), (x'_i, -x'_r).
-template <class F>
-__global__ __launch_bounds__(kT) void gemv_vectors_kernel(int64_t n, const F* x, int64_t incx, F al_r, F al_i,
-                                                          int mode, int conj, F* v1, F* v2) {
-  const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (j >= n) return;
-  const F* p = x + 2 * j * incx;
-  F xr, xi;
-  times_scalar(mode, p[0], p[1], al_r, al_i, xr, xi);
-  if (!conj) {
-    v1[2 * j] = xr;
-    v1[2 * j + 1] = -xi;
-    v2[2 * j] = xi;
-    v2[2 * j + 1] = xr;
-  } else {
-    v1[2 * j] = xr;
-    v1[2 * j + 1] = xi;
-    v2[2 * j] = xi;
-    v2[2 * j + 1] = -xr;
-  }
-}
-
-// alpha*x gathered contiguously (interleaved) for the column path.
+// alpha*x gathered contiguously (interleaved) for the gemv kernels.
 template <class F>
 __global__ __launch_bounds__(kT) void alpha_x_kernel(int64_t n, const F* x, int64_t incx, F al_r, F al_i, int mode,
                                                      F* xs) {
@@ -252,20 +226,6 @@ bool real_gemm<float>(int fold, int64_t M, int64_t N, int64_t K, const float* A,
                       hipStream_t s) {
   return sbsgemm(fold, 'R', 'N', 'N', M, N, K, 1.0f, A, K, B, N, Cb, N, s);
 }
-template <class F>
-bool real_gemv(int fold, int64_t M, int64_t N, const F* A, int64_t lda, const F* x, F* Yb, int64_t incYb,
-               hipStream_t s);
-template <>
-bool real_gemv<double>(int fold, int64_t M, int64_t N, const double* A, int64_t lda, const double* x, double* Yb,
-                       int64_t incYb, hipStream_t s) {
-  return dbdgemv(fold, 'R', 'N', M, N, 1.0, A, lda, x, 1, Yb, incYb, s);
-}
-template <>
-bool real_gemv<float>(int fold, int64_t M, int64_t N, const float* A, int64_t lda, const float* x, float* Yb,
-                      int64_t incYb, hipStream_t s) {
-  return sbsgemv(fold, 'R', 'N', M, N, 1.0f, A, lda, x, 1, Yb, incYb, s);
-}
-
 template <class F>
 bool fold_ok(int fold) {
   if (fold >= 2 && fold <= Fmt<F>::max_fold) return true;
@@ -320,19 +280,12 @@ bool zgemv_into(int fold, char order, char trans, int64_t M, int64_t N, const F*
   // two real row-dot passes over the interleaved rows; otherwise the terms of
   // an output are lda apart: one complex column pass
   const bool rows_contig = rm ? (tc == 0) : (tc != 0);
-  if (!rows_contig) {
-    F* xs = (F*)workspace(s, kSlotZB, sizeof(F) * (size_t)(2 * No));
-    if (!xs) return false;
-    alpha_x_kernel<F><<<grid1(No), kT, 0, s>>>(No, x, incx, alpha[0], alpha[1], mode, xs);
-    if (!RB_HIP(hipGetLastError())) return false;
-    return zgemv_cols(fold, Mo, No, A, lda, conj, xs, R, s);
-  }
-  F* V = (F*)workspace(s, kSlotZB, sizeof(F) * (size_t)(4 * No));
-  if (!V) return false;
-  gemv_vectors_kernel<F><<<grid1(No), kT, 0, s>>>(No, x, incx, alpha[0], alpha[1], mode, conj, V, V + 2 * No);
+  F* xs = (F*)workspace(s, kSlotZB, sizeof(F) * (size_t)(2 * No));
+  if (!xs) return false;
+  alpha_x_kernel<F><<<grid1(No), kT, 0, s>>>(No, x, incx, alpha[0], alpha[1], mode, xs);
   if (!RB_HIP(hipGetLastError())) return false;
-  return real_gemv<F>(fold, Mo, 2 * No, A, 2 * lda, V, R, 2, s) &&
-         real_gemv<F>(fold, Mo, 2 * No, A, 2 * lda, V + 2 * No, R + 2 * fold, 2, s);
+  if (rows_contig) return zgemv_rows(fold, Mo, No, A, lda, conj, xs, R, s);
+  return zgemv_cols(fold, Mo, No, A, lda, conj, xs, R, s);
 }
 
 template <class F>

@@ -112,6 +112,13 @@ bool zgemv_cols(int fold, int64_t Mo, int64_t No, const double* A, int64_t lda, 
 bool zgemv_cols(int fold, int64_t Mo, int64_t No, const float* A, int64_t lda, int conj, const float* xs, float* R,
                 hipStream_t s);
 
+// The same for outputs whose terms are contiguous (row-major N, column-major
+// T/C): op(A)_ik = A[i*lda + k], one complex dot per row, one pass over A.
+bool zgemv_rows(int fold, int64_t Mo, int64_t No, const double* A, int64_t lda, int conj, const double* xs,
+                double* R, hipStream_t s);
+bool zgemv_rows(int fold, int64_t Mo, int64_t No, const float* A, int64_t lda, int conj, const float* xs, float* R,
+                hipStream_t s);
+
 // Complex gemm / gemv on device operands (interleaved pairs; alpha and beta
 // are host pairs): binnedBLAS_zbzgemm / zbzgemv semantics into complex
 // binned arrays (Zb with leading dimension ldz, Yb with increment incy, in
