@@ -21,8 +21,10 @@ SOURCES = [
     "runtime.hip",
     "host_binned.hip",
     "binned_vec.hip",
-    "blas1.hip",
-    "complex.hip",
+    "blas1_d.hip",
+    "blas1_f.hip",
+    "complex_d.hip",
+    "complex_f.hip",
     "complex_mat.hip",
     "amax.hip",
     "gemv.hip",

@@ -1,4 +1,4 @@
-// blas1.hip -- single-pass binned reductions over device vectors:
+// blas1_impl.hpp -- single-pass binned reductions over device vectors:
 // sum, asum, dot and the scaled sum of squares behind nrm2.
 //
 // Reference path: binnedBLAS_dmdsum (src/binnedBLAS/dmdsum.ccog:57-89),
@@ -16,6 +16,7 @@
 // Why this equals the reference bit for bit: the slice each bin receives from
 // a term is independent of order and of the accumulator, the index ends at
 // dindex(max |term|) either way, and the merges are exact.
+#pragma once
 #include <float.h>
 #include <limits.h>
 
@@ -316,8 +317,6 @@ bool launch_seg_term(Term term, const SegArgs& a, int grid, hipStream_t s) {
 
 }  // namespace
 
-bool fold_supported(int fold) { return fold >= 2 && fold <= kMaxFold; }
-
 namespace {
 template <class F>
 bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const F* y, int64_t incy,
@@ -373,46 +372,5 @@ bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const 
   }
 }
 }  // namespace
-
-bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
-           const double* state_in, double scale_in, const double* scale_in_dev, double* state_out, double* scale_out,
-           double* result, hipStream_t s) {
-  return blas1_impl<double>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out, scale_out,
-                            result, s);
-}
-
-bool blas1_s(int fold, Term term, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy,
-             const float* state_in, float scale_in, const float* scale_in_dev, float* state_out, float* scale_out,
-             float* result, hipStream_t s) {
-  return blas1_impl<float>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out, scale_out,
-                           result, s);
-}
-
-bool segments(int fold, Term term, const double* x, const double* y, const SegChunk* chunks, int64_t nchunks,
-              double* partials, const int64_t* first_chunk, int64_t nseg, double* results, int grid, hipStream_t s) {
-  if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
-    return false;
-  }
-  SegArgs a;
-  a.x = x;
-  a.y = y;
-  a.chunks = chunks;
-  a.nchunks = nchunks;
-  a.partials = partials;
-  a.first_chunk = first_chunk;
-  a.nseg = nseg;
-  a.results = results;
-  a.fold = fold;
-  if (grid < 1) grid = 1;
-  switch (fold) {
-    case 2: return launch_seg_term<2, true>(term, a, grid, s);
-    case 3: return launch_seg_term<3, true>(term, a, grid, s);
-    case 4: return launch_seg_term<4, true>(term, a, grid, s);
-    default:
-      if (fold <= 8) return launch_seg_term<8, false>(term, a, grid, s);
-      return launch_seg_term<kMaxFold, false>(term, a, grid, s);
-  }
-}
 
 }  // namespace rb

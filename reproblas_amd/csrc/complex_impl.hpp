@@ -1,4 +1,4 @@
-// complex.hip -- single-pass binned reductions over complex device vectors:
+// complex_impl.hpp -- single-pass binned reductions over complex device vectors:
 // sum, dotu, dotc (complex binned results) and asum, the scaled sum of
 // squares behind nrm2 (real binned results), for double complex and float
 // complex data (interleaved real, imaginary pairs).
@@ -20,6 +20,7 @@
 // registers; each channel's index (and scale) is common to the wave, as in
 // the real kernels (wave_sum.hpp), and all merges are exact, so the bits do
 // not depend on the tiling.
+#pragma once
 #include <float.h>
 
 #include <algorithm>
@@ -35,6 +36,18 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
+
+// Unit-stride workgroups per CU of the complex reductions and persistent
+// workgroups per CU of the complex row-dot gemv (compile-time overridable for
+// tuning builds). Measured on MI355X (2^27 double complex, fold 3): 1/CU zsum
+// 6.0, dznrm2 5.5, zdotc 6.2 TB/s; 2/CU 6.2, 6.1, 6.5 TB/s; the row-dot gemv
+// is within noise at 2, 4 and 8.
+#ifndef RB_ZBLAS1_BLOCKS_PER_CU
+#define RB_ZBLAS1_BLOCKS_PER_CU 2
+#endif
+#ifndef RB_ZROW_BLOCKS_PER_CU
+#define RB_ZROW_BLOCKS_PER_CU 4
+#endif
 
 template <class F>
 struct ZArgs {
@@ -518,12 +531,12 @@ bool zlaunch_rows(const ZRowArgs<F>& a, bool conj, int mode, int blocks, hipStre
     if (mode == 0)
       zrowdot_kernel<CAP, FIXED, true, 0, F><<<blocks, kThreads, 0, s>>>(a);
     else
-      zrowdot_kernel<CAP, FIXED, true, 1, F><<<blocks, kThreads, 0, s>>>(a);
+      zrowdot_kernel<CAP, FIXED, true, 2, F><<<blocks, kThreads, 0, s>>>(a);
   } else {
     if (mode == 0)
       zrowdot_kernel<CAP, FIXED, false, 0, F><<<blocks, kThreads, 0, s>>>(a);
     else
-      zrowdot_kernel<CAP, FIXED, false, 1, F><<<blocks, kThreads, 0, s>>>(a);
+      zrowdot_kernel<CAP, FIXED, false, 2, F><<<blocks, kThreads, 0, s>>>(a);
   }
   return RB_HIP(hipGetLastError());
 }
@@ -546,25 +559,22 @@ bool zrows_impl(int fold, int64_t Mo, int64_t No, const F* A, int64_t lda, int c
   const bool aligned = ((uintptr_t)A % 16 == 0) && ((uintptr_t)xs % 16 == 0) &&
                        ((2 * sizeof(F) * (size_t)lda) % 16 == 0);
   const int mode = aligned ? 0 : 1;
-  const int64_t blocks = std::min<int64_t>((Mo + kWaves - 1) / kWaves, (int64_t)dev->cus * 4);
-  switch (fold) {
-    case 2: return zlaunch_rows<2, true>(a, conj, mode, (int)blocks, s);
-    case 3: return zlaunch_rows<3, true>(a, conj, mode, (int)blocks, s);
-    case 4: return zlaunch_rows<4, true>(a, conj, mode, (int)blocks, s);
-    default:
-      if (fold <= 8) return zlaunch_rows<8, false>(a, conj, mode, (int)blocks, s);
-      return zlaunch_rows<Fmt<F>::max_fold, false>(a, conj, mode, (int)blocks, s);
-  }
+  const int64_t blocks = std::min<int64_t>((Mo + kWaves - 1) / kWaves, (int64_t)dev->cus * RB_ZROW_BLOCKS_PER_CU);
+  // fold 3 (the default) has a dedicated instance; the others run the
+  // general ones (register slots up to 8, then private memory)
+  if (fold == 3) return zlaunch_rows<3, true>(a, conj, mode, (int)blocks, s);
+  if (fold <= 8) return zlaunch_rows<8, false>(a, conj, mode, (int)blocks, s);
+  return zlaunch_rows<Fmt<F>::max_fold, false>(a, conj, mode, (int)blocks, s);
 }
 
 template <int CAP, bool FIXED, int T, class F>
 bool zlaunch_mode(int mode, const ZArgs<F>& a, int blocks, hipStream_t s) {
   constexpr int U = z_units<T, F>();
-  switch (mode) {
-    case 0: zblas1_kernel<CAP, FIXED, T, U, 0, F><<<blocks, kThreads, 0, s>>>(a); break;
-    case 1: zblas1_kernel<CAP, FIXED, T, U, 1, F><<<blocks, kThreads, 0, s>>>(a); break;
-    default: zblas1_kernel<CAP, FIXED, T, U, 2, F><<<blocks, kThreads, 0, s>>>(a); break;
-  }
+  // unaligned unit-stride operands take the element-load (strided) kernel
+  if (mode == 0)
+    zblas1_kernel<CAP, FIXED, T, U, 0, F><<<blocks, kThreads, 0, s>>>(a);
+  else
+    zblas1_kernel<CAP, FIXED, T, U, 2, F><<<blocks, kThreads, 0, s>>>(a);
   return RB_HIP(hipGetLastError());
 }
 
@@ -618,49 +628,20 @@ bool zblas1_impl(int fold, ZTerm term, int64_t n, const F* x, int64_t incx, cons
     const bool aligned = ((uintptr_t)x % 16 == 0) && (!dot || (uintptr_t)y % 16 == 0);
     mode = aligned ? 0 : 1;
   }
-  // grid: one workgroup per CU for unit stride (as the real kernels), four for
-  // strided operands; at least 2048 terms per wave for short vectors
+  // grid: RB_ZBLAS1_BLOCKS_PER_CU workgroups per CU for aligned unit stride,
+  // four otherwise; at least 2048 terms per wave for short vectors
   const int64_t min_tiles = std::max<int64_t>(1, 2048 / (per_tile * (dot ? 2 : 1)));
   int64_t blocks = (tiles + kWaves * min_tiles - 1) / (kWaves * min_tiles);
-  const int64_t cap = (int64_t)dev->cus * (mode == 2 ? 4 : 1);
+  const int64_t cap = (int64_t)dev->cus * (mode == 2 ? 4 : RB_ZBLAS1_BLOCKS_PER_CU);
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
   a.partials = (F*)workspace(s, kSlotPartials, sizeof(F) * (size_t)blocks * 2 * (2 * fold + 1));
   a.ticket = tickets(s, 1);
   if (!a.partials || !a.ticket) return false;
-  switch (fold) {
-    case 2: return zlaunch_term<2, true>(term, mode, a, (int)blocks, s);
-    case 3: return zlaunch_term<3, true>(term, mode, a, (int)blocks, s);
-    case 4: return zlaunch_term<4, true>(term, mode, a, (int)blocks, s);
-    default:
-      if (fold <= 8) return zlaunch_term<8, false>(term, mode, a, (int)blocks, s);
-      return zlaunch_term<Fmt<F>::max_fold, false>(term, mode, a, (int)blocks, s);
-  }
+  if (fold == 3) return zlaunch_term<3, true>(term, mode, a, (int)blocks, s);
+  if (fold <= 8) return zlaunch_term<8, false>(term, mode, a, (int)blocks, s);
+  return zlaunch_term<Fmt<F>::max_fold, false>(term, mode, a, (int)blocks, s);
 }
 
 }  // namespace
-
-bool zblas1(int fold, ZTerm term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
-            const double* state_in, double scale_in, const double* scale_in_dev, double* state_out,
-            double* scale_out, double* result, hipStream_t s) {
-  return zblas1_impl<double>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out,
-                             scale_out, result, s);
-}
-
-bool zblas1(int fold, ZTerm term, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy,
-            const float* state_in, float scale_in, const float* scale_in_dev, float* state_out, float* scale_out,
-            float* result, hipStream_t s) {
-  return zblas1_impl<float>(fold, term, n, x, incx, y, incy, state_in, scale_in, scale_in_dev, state_out,
-                            scale_out, result, s);
-}
-
-bool zgemv_rows(int fold, int64_t Mo, int64_t No, const double* A, int64_t lda, int conj, const double* xs,
-                double* R, hipStream_t s) {
-  return zrows_impl<double>(fold, Mo, No, A, lda, conj, xs, R, s);
-}
-bool zgemv_rows(int fold, int64_t Mo, int64_t No, const float* A, int64_t lda, int conj, const float* xs, float* R,
-                hipStream_t s) {
-  return zrows_impl<float>(fold, Mo, No, A, lda, conj, xs, R, s);
-}
-
 }  // namespace rb

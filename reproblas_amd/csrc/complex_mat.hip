@@ -16,7 +16,7 @@
 // outputs are the real (columns 0..N-1) and imaginary (N..2N-1) parts, each
 // a binned sum over exactly the reference's terms: the real kernels' bits
 // equal the reference's. gemv reads A once: with contiguous rows of op(A) a
-// complex row-dot kernel (complex.hip, zgemv_rows), with terms lda apart the
+// complex row-dot kernel (complex_impl.hpp, zgemv_rows), with terms lda apart the
 // complex mode of the column gemv (gemv.hip, zgemv_cols).
 #include <algorithm>
 
