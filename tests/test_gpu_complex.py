@@ -441,3 +441,46 @@ def test_gemv_columns_unaligned_matrix(gpu, prec):
                                                            ctypes.c_void_p(tx.data_ptr()), 1, O.p(be),
                                                            ctypes.c_void_p(ty.data_ptr()), 1, s) == 0
     assert bits_eq(ty.cpu().numpy(), iv(ye))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_matches_reference_fixtures(gpu, prec):
+    """GPU results equal the compiled reference's own complex outputs stored in
+    tests/golden/sc.npz (BLAS1 over eight fills, gemv, all gemm layouts)."""
+    import os
+
+    import reproblas_amd as rb
+
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sc.npz"))
+    for k in [k for k in G.files if k.startswith(prec + "_") and k.endswith("_res")]:
+        parts = k.split("_")
+        kind, n, fold = "_".join(parts[1:-3]), parts[-3], int(parts[-2])
+        x, y = G[f"{prec}_{kind}_{n}_x"].view(cdt(prec)), G[f"{prec}_{kind}_{n}_y"].view(cdt(prec))
+        got = np.concatenate([iv(gpu_op(rb, prec, "sum", fold, x)), iv(gpu_op(rb, prec, "dotu", fold, x, y)),
+                              iv(gpu_op(rb, prec, "dotc", fold, x, y)),
+                              np.array([gpu_op(rb, prec, "asum", fold, x), gpu_op(rb, prec, "nrm2", fold, x)])])
+        assert bits_eq(got, G[k]), k
+    A, B, C = G[f"{prec}_mat_A"], G[f"{prec}_mat_B"], G[f"{prec}_mat_C"]
+    dt = A.dtype
+    M, N, K = 23, 17, 200
+    al, be = np.array([1.5, -0.5], dt), np.array([0.5, 0.25], dt)
+    L = rb.lib()
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    for order in "CR":
+        for ta in "NTC":
+            if not (order == "C" and ta == "C"):
+                y = C.copy()
+                lda = (M if order == "C" else N) + 1
+                getattr(L, f"reproBLAS_r{prec}gemv")(3, order.encode(), ta.encode(), M, N, P(al), P(A), lda, P(B), 1,
+                                                     P(be), P(y), 1)
+                rb._check()
+                assert bits_eq(y, G[f"{prec}_gemv_{order}{ta}"]), (order, ta)
+            for tb in "NTC":
+                rm = order == "R"
+                la = (K if ta == "N" else M) if rm else (M if ta == "N" else K)
+                lb = (N if tb == "N" else K) if rm else (K if tb == "N" else N)
+                c = C.copy()
+                getattr(L, f"reproBLAS_r{prec}gemm")(3, order.encode(), ta.encode(), tb.encode(), M, N, K, P(al), P(A),
+                                                     la, P(B), lb, P(be), P(c), N if rm else M)
+                rb._check()
+                assert bits_eq(c, G[f"{prec}_gemm_{order}{ta}{tb}"]), (order, ta, tb)

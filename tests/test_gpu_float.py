@@ -491,3 +491,38 @@ def test_full_size_sdot_snrm2(gpu, fold):
     assert same(rb.rsdot(tx, ty, fold), O.rsdot(fold, x, y))
     assert same(rb.rsnrm2(tx, fold), O.rsnrm2(fold, x))
     assert same(rb.rssum(tx, fold), O.rssum(fold, x))
+
+
+def test_matches_reference_fixtures(gpu):
+    """GPU results equal the compiled reference's own outputs stored in
+    tests/golden/sc.npz (single precision BLAS1, gemv, gemm)."""
+    import os
+
+    import reproblas_amd as rb
+
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sc.npz"))
+    for k in [k for k in G.files if k.startswith("s_") and k.endswith("_res")]:
+        parts = k.split("_")
+        kind, n, fold = "_".join(parts[1:-3]), parts[-3], int(parts[-2])
+        x, y = G[f"s_{kind}_{n}_x"], G[f"s_{kind}_{n}_y"]
+        got = np.array([rb.rssum(x, fold), rb.rsasum(x, fold), rb.rsnrm2(x, fold), rb.rsdot(x, y, fold)], np.float32)
+        assert eq_arrays(got, G[k]), k
+    A, B, C0 = G["s_mat_A"], G["s_mat_B"], G["s_mat_C"]
+    M, N, K = 37, 29, 300
+    L = rb.lib()
+    for order in "CR":
+        for tr in "NT":
+            y = C0.copy()
+            lda = (M if order == "C" else N) + 1
+            L.reproBLAS_rsgemv(3, order.encode(), tr.encode(), M, N, 1.5, vp(A), lda, vp(B), 1, 0.5, vp(y), 1)
+            rb._check()
+            assert eq_arrays(y, G[f"s_gemv_{order}{tr}"]), (order, tr)
+            for tb in "NT":
+                rm = order == "R"
+                la = (K if tr == "N" else M) if rm else (M if tr == "N" else K)
+                lb = (N if tb == "N" else K) if rm else (K if tb == "N" else N)
+                c = C0.copy()
+                L.reproBLAS_rsgemm(3, order.encode(), tr.encode(), tb.encode(), M, N, K, 0.75, vp(A), la, vp(B), lb,
+                                   0.5, vp(c), N if rm else M)
+                rb._check()
+                assert eq_arrays(c, G[f"s_gemm_{order}{tr}{tb}"]), (order, tr, tb)

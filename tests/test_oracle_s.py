@@ -233,3 +233,47 @@ def test_closed_forms_and_round_trips():
             Y = np.zeros(2 * fold, np.float32)
             O.fn["sbsconv"](fold, t, vp(Y))
             assert same(O.ssbconv(fold, Y), t), (fold, t)
+
+
+# ---------------------------------------------------------------- golden --
+def _golden_sc():
+    import os
+
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sc.npz")
+    return np.load(p)
+
+
+def test_oracle_matches_golden_fixtures():
+    """tests/golden/sc.npz holds the compiled reference's single-precision
+    outputs (tests/golden/make_golden_sc.py); the oracle reproduces them
+    without the reference library."""
+    O = oracle_s()
+    G = _golden_sc()
+    keys = [k for k in G.files if k.startswith("s_") and k.endswith("_res")]
+    assert keys
+    for k in keys:
+        parts = k.split("_")
+        kind, n, fold = "_".join(parts[1:-3]), parts[-3], int(parts[-2])
+        x, y = G[f"s_{kind}_{n}_x"], G[f"s_{kind}_{n}_y"]
+        got = np.array([O.rssum(fold, x), O.rsasum(fold, x), O.rsnrm2(fold, x), O.rsdot(fold, x, y)], np.float32)
+        assert eq_arrays(got, G[k]), k
+        st = np.zeros(2 * fold, np.float32)
+        O.sbssum(fold, x, st)
+        if not np.isnan(st[0]):
+            assert eq_arrays(st, G[f"s_{kind}_{n}_{fold}_sbssum"]), k
+    A, B, C0 = G["s_mat_A"], G["s_mat_B"], G["s_mat_C"]
+    M, N, K = 37, 29, 300
+    for order in "CR":
+        for tr in "NT":
+            y = C0.copy()
+            lda = (M if order == "C" else N) + 1
+            O.fn["rsgemv"](3, order.encode(), tr.encode(), M, N, 1.5, vp(A), lda, vp(B), 1, 0.5, vp(y), 1)
+            assert eq_arrays(y, G[f"s_gemv_{order}{tr}"]), (order, tr)
+            for tb in "NT":
+                rm = order == "R"
+                la = (K if tr == "N" else M) if rm else (M if tr == "N" else K)
+                lb = (N if tb == "N" else K) if rm else (K if tb == "N" else N)
+                c = C0.copy()
+                O.fn["rsgemm"](3, order.encode(), tr.encode(), tb.encode(), M, N, K, 0.75, vp(A), la, vp(B), lb, 0.5,
+                               vp(c), N if rm else M)
+                assert eq_arrays(c, G[f"s_gemm_{order}{tr}{tb}"]), (order, tr, tb)

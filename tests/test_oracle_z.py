@@ -260,3 +260,43 @@ def test_conjugation_identities(prec):
         yp = y.reshape(-1, 2)[perm].ravel().copy()
         assert bits_eq(O.rsum(fold, xp), O.rsum(fold, x))
         assert bits_eq(O.rdot("u", fold, xp, yp), O.rdot("u", fold, x, y))
+
+
+# ---------------------------------------------------------------- golden --
+@pytest.mark.parametrize("prec", PRECS)
+def test_oracle_matches_golden_fixtures(prec):
+    """tests/golden/sc.npz holds the compiled reference's complex outputs
+    (tests/golden/make_golden_sc.py); the oracle reproduces them without the
+    reference library."""
+    import os
+
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sc.npz"))
+    O = oracle_z(prec)
+    keys = [k for k in G.files if k.startswith(prec + "_") and k.endswith("_res")]
+    assert keys
+    for k in keys:
+        parts = k.split("_")
+        kind, n, fold = "_".join(parts[1:-3]), parts[-3], int(parts[-2])
+        x, y = G[f"{prec}_{kind}_{n}_x"], G[f"{prec}_{kind}_{n}_y"]
+        got = np.concatenate([O.rsum(fold, x), O.rdot("u", fold, x, y), O.rdot("c", fold, x, y),
+                              np.array([O.rasum(fold, x), O.rnrm2(fold, x)], x.dtype)])
+        assert bits_eq(got, G[k]), k
+    A, B, C = G[f"{prec}_mat_A"], G[f"{prec}_mat_B"], G[f"{prec}_mat_C"]
+    dt = A.dtype
+    M, N, K = 23, 17, 200
+    al, be = np.array([1.5, -0.5], dt), np.array([0.5, 0.25], dt)
+    for order in "CR":
+        for ta in "NTC":
+            if not (order == "C" and ta == "C"):
+                y = C.copy()
+                lda = (M if order == "C" else N) + 1
+                O.fn["rgemv"](3, order.encode(), ta.encode(), M, N, O.p(al), O.p(A), lda, O.p(B), 1, O.p(be), O.p(y), 1)
+                assert bits_eq(y, G[f"{prec}_gemv_{order}{ta}"]), (order, ta)
+            for tb in "NTC":
+                rm = order == "R"
+                la = (K if ta == "N" else M) if rm else (M if ta == "N" else K)
+                lb = (N if tb == "N" else K) if rm else (K if tb == "N" else N)
+                c = C.copy()
+                O.fn["rgemm"](3, order.encode(), ta.encode(), tb.encode(), M, N, K, O.p(al), O.p(A), la, O.p(B), lb,
+                              O.p(be), O.p(c), N if rm else M)
+                assert bits_eq(c, G[f"{prec}_gemm_{order}{ta}{tb}"]), (order, ta, tb)
