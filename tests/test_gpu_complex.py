@@ -484,3 +484,19 @@ def test_matches_reference_fixtures(gpu, prec):
                                                      la, P(B), lb, P(be), P(c), N if rm else M)
                 rb._check()
                 assert bits_eq(c, G[f"{prec}_gemm_{order}{ta}{tb}"]), (order, ta, tb)
+
+
+@pytest.mark.slow
+def test_full_size_zdotc_zsum_dznrm2(gpu):
+    """Benchmark scale: N = 2^27 double complex wide-range zdotc, zsum and
+    dznrm2, bit-identical to the oracle."""
+    import torch
+
+    import reproblas_amd as rb
+
+    n = 1 << 27
+    x, y = cvec("z", "wide", n, 501), cvec("z", "wide", n, 502)
+    tx, ty = torch.from_numpy(x).to(gpu), torch.from_numpy(y).to(gpu)
+    assert bits_eq(gpu_op(rb, "z", "dotc", 3, tx, ty), orc_op("z", "dotc", 3, x, y))
+    assert bits_eq(gpu_op(rb, "z", "sum", 3, tx), orc_op("z", "sum", 3, x))
+    assert bits_eq(gpu_op(rb, "z", "nrm2", 3, tx), orc_op("z", "nrm2", 3, x))
