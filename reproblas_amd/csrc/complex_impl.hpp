@@ -473,10 +473,26 @@ RB_UNROLL_CAP
   }
 }
 
+// Complex elements per lane per tile of the row-dot gemv (compile-time
+// overridable for tuning builds). 32 bytes of each operand per lane: on
+// MI355X, zgemv 8192^2 column-major T/C at fold 3 runs 4.9-5.0 TB/s with 2
+// double complex per lane against 4.4-4.8 with 4 and 3.8-3.9 with 8 (the
+// smaller tile leaves room for more waves per SIMD).
+#ifndef RB_ZROW_U_D
+#define RB_ZROW_U_D 2
+#endif
+#ifndef RB_ZROW_U_F
+#define RB_ZROW_U_F 4
+#endif
+template <class F>
+constexpr int zrow_units() {
+  return sizeof(F) == sizeof(double) ? RB_ZROW_U_D : RB_ZROW_U_F;
+}
+
 template <int CAP, bool FIXED, bool CONJ, int MODE, class F>
 __global__ __launch_bounds__(kThreads) void zrowdot_kernel(ZRowArgs<F> a) {
   constexpr int T = CONJ ? kZDotC : kZDotU;
-  constexpr int U = z_units<T, F>();
+  constexpr int U = zrow_units<F>();
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
   const int64_t nw = (int64_t)gridDim.x * kWaves;
