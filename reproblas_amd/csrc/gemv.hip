@@ -106,10 +106,15 @@ __global__ __launch_bounds__(kThreads) void scale_x_kernel(GemvArgs<F> a, F* xs)
   if (k < a.len) xs[k] = scaled_x(a, k);
 }
 
-// Terms per lane per tile of the row-dot (the BLAS1 dot's shape: 8 KiB per
-// wave tile).
+// Terms per lane per tile of the row-dot (compile-time overridable for
+// tuning builds; default the BLAS1 dot's shape: 8 KiB per wave tile). On
+// MI355X, dgemv / sgemv 16384^2 column-major T at fold 3: 8 doubles per lane
+// 5.8 / 5.9 TB/s, 4 doubles 5.5 / 4.9, 2 doubles 5.6 / 5.5.
+#ifndef RB_GEMV_ROW_U
+#define RB_GEMV_ROW_U 8
+#endif
 template <class F>
-constexpr int u_row() { return 8 * (int)(sizeof(double) / sizeof(F)); }
+constexpr int u_row() { return RB_GEMV_ROW_U * (int)(sizeof(double) / sizeof(F)); }
 
 template <class F>
 __device__ __forceinline__ Span<F> row_span(const GemvArgs<F>& a, const F* xs, int64_t o) {
