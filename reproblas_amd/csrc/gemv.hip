@@ -174,6 +174,74 @@ __global__ __launch_bounds__(kThreads) void rowdot_kernel(GemvArgs<F> a, const F
   }
 }
 
+// Rows 2p and 2p + 1 of tile t: A tiles of both, the xs tile once.
+template <int MODE, class F>
+__device__ __forceinline__ void load_row_pair(const GemvArgs<F>& a, const F* xs, int64_t p, int64_t t, int lane,
+                                              F (&v0)[u_row<F>()], F (&v1)[u_row<F>()], F (&vx)[u_row<F>()]) {
+  constexpr int kURow = u_row<F>();
+  load_tile<kDot, kURow, MODE>(row_span(a, xs, 2 * p), t, lane, v0, vx);
+  if (2 * p + 1 < a.nout) {
+    F unused[kURow];
+    load_tile<kSum, kURow, MODE>(row_span(a, xs, 2 * p + 1), t, lane, v1, unused);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kURow; ++k) v1[k] = F(0);
+  }
+}
+
+// Row pairs: wave g of W owns pairs g, g + W, ... (rows 2p and 2p + 1) and
+// deposits both rows of a tile against one xs tile, so each xs element is
+// read once per two rows. Each row keeps its own bins and wave index.
+template <int CAP, bool FIXED, int MODE, class F>
+__global__ __launch_bounds__(kThreads) void rowdot2_kernel(GemvArgs<F> a, const F* xs) {
+  constexpr int kURow = u_row<F>();
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * kWaves;
+  const int64_t pairs = (a.nout + 1) / 2;
+  int64_t p = (int64_t)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (p >= pairs) return;  // whole wave leaves together
+  const int64_t tiles = (a.len + 64 * kURow - 1) / (64 * kURow);
+  State<CAP, F> s0, s1;
+  WaveRun<F> w0, w1;
+  wave_start(fold, w0, s0);
+  wave_start(fold, w1, s1);
+  F a0[kURow], a1[kURow], ax[kURow], b0[kURow], b1[kURow], bx[kURow];
+  int64_t t = 0;
+  load_row_pair<MODE>(a, xs, p, t, lane, a0, a1, ax);
+  while (true) {
+    int64_t pn = p, tn = t + 1;
+    if (tn == tiles) { pn = p + nw; tn = 0; }
+    if (pn < pairs) load_row_pair<MODE>(a, xs, pn, tn, lane, b0, b1, bx);
+    deposit_tile<CAP, kDot, kURow>(fold, a0, ax, w0, s0);
+    deposit_tile<CAP, kDot, kURow>(fold, a1, ax, w1, s1);
+    if (tn == 0) {
+      finish_row<CAP, MODE>(a, xs, fold, 2 * p, lane, w0, s0);
+      if (2 * p + 1 < a.nout) finish_row<CAP, MODE>(a, xs, fold, 2 * p + 1, lane, w1, s1);
+      wave_start(fold, w0, s0);
+      wave_start(fold, w1, s1);
+    }
+    if (pn >= pairs) break;
+    p = pn;
+    t = tn;
+    pn = p;
+    tn = t + 1;
+    if (tn == tiles) { pn = p + nw; tn = 0; }
+    if (pn < pairs) load_row_pair<MODE>(a, xs, pn, tn, lane, a0, a1, ax);
+    deposit_tile<CAP, kDot, kURow>(fold, b0, bx, w0, s0);
+    deposit_tile<CAP, kDot, kURow>(fold, b1, bx, w1, s1);
+    if (tn == 0) {
+      finish_row<CAP, MODE>(a, xs, fold, 2 * p, lane, w0, s0);
+      if (2 * p + 1 < a.nout) finish_row<CAP, MODE>(a, xs, fold, 2 * p + 1, lane, w1, s1);
+      wave_start(fold, w0, s0);
+      wave_start(fold, w1, s1);
+    }
+    if (pn >= pairs) break;
+    p = pn;
+    t = tn;
+  }
+}
+
 // --------------------------------------------------------------- col-axpy --
 // Block: 128 outputs (2 per lane) x one column chunk; wave w takes columns
 // chunk_start + w, +4, +8, ...
@@ -394,6 +462,17 @@ RB_UNROLL_CAP
 #define RB_GEMV_ROW_BLOCKS_PER_CU 4
 #endif
 constexpr int kRowBlocksPerCU = RB_GEMV_ROW_BLOCKS_PER_CU;
+// 1: the row-dot walks rows in pairs (rowdot2_kernel), 0: one row per wave.
+// On MI355X, 16384^2 column-major T at fold 3: double 5.8 -> 6.4 TB/s with
+// pairs, float 5.9 -> 5.75 TB/s (pairs off).
+#ifndef RB_GEMV_ROW_PAIR_D
+#define RB_GEMV_ROW_PAIR_D 1
+#endif
+#ifndef RB_GEMV_ROW_PAIR_F
+#define RB_GEMV_ROW_PAIR_F 0
+#endif
+template <class F>
+constexpr bool row_pair() { return sizeof(F) == 8 ? RB_GEMV_ROW_PAIR_D : RB_GEMV_ROW_PAIR_F; }
 
 template <int CAP, bool FIXED, class F, bool CPLX>
 bool launch_cols(GemvArgs<F>& a, hipStream_t s);
@@ -406,12 +485,21 @@ bool launch_gemv(bool rowdot, GemvArgs<F>& a, hipStream_t s) {
     F* xs = (F*)workspace(s, kSlotGemvX, sizeof(F) * (size_t)a.len);
     if (!xs) return false;
     scale_x_kernel<<<(unsigned)((a.len + kThreads - 1) / kThreads), kThreads, 0, s>>>(a, xs);
-    const int64_t blocks = std::min<int64_t>((a.nout + kWaves - 1) / kWaves, (int64_t)d->cus * kRowBlocksPerCU);
     const bool vec = ((uintptr_t)a.A % 16 == 0) && (a.lda % 2 == 0);
-    if (vec)
-      rowdot_kernel<CAP, FIXED, 0, F><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
-    else
-      rowdot_kernel<CAP, FIXED, 1, F><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
+    if constexpr (row_pair<F>()) {
+      const int64_t pairs = (a.nout + 1) / 2;
+      const int64_t blocks = std::min<int64_t>((pairs + kWaves - 1) / kWaves, (int64_t)d->cus * kRowBlocksPerCU);
+      if (vec)
+        rowdot2_kernel<CAP, FIXED, 0, F><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
+      else
+        rowdot2_kernel<CAP, FIXED, 1, F><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
+    } else {
+      const int64_t blocks = std::min<int64_t>((a.nout + kWaves - 1) / kWaves, (int64_t)d->cus * kRowBlocksPerCU);
+      if (vec)
+        rowdot_kernel<CAP, FIXED, 0, F><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
+      else
+        rowdot_kernel<CAP, FIXED, 1, F><<<(unsigned)blocks, kThreads, 0, s>>>(a, xs);
+    }
     return RB_HIP(hipGetLastError());
   }
   return launch_cols<CAP, FIXED, F, false>(a, s);
