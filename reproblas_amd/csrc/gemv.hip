@@ -11,7 +11,8 @@
 //   row-dot  -- the terms of an output are contiguous (row-major N,
 //               column-major T): fl(alpha*x) is first gathered into a
 //               contiguous vector; then a persistent grid of waves, one
-//               output per wave at a time, streams the rows as BLAS1 dots
+//               output (double: a pair of outputs sharing the x tiles) per
+//               wave at a time, streams the rows as BLAS1 dots
 //               (wave_sum.hpp) and one lane folds in beta*y and converts.
 //   col-axpy -- the terms of an output are lda apart (column-major N,
 //               row-major T): each lane owns two adjacent outputs and walks
@@ -261,9 +262,24 @@ __device__ __forceinline__ void complex_terms(F ar, F ai, F xr, F xi, int conj, 
   i1 = ar * xi;
 }
 
+// Columns per step of the col-axpy (real / complex) and its workgroups per
+// CU (compile-time overridable for tuning builds). On MI355X, column-major N
+// at fold 3, zgemv 8192^2 / dgemv 16384^2: the defaults 4.95 / 6.0 TB/s;
+// complex 2 columns 5.0, 8 columns 4.6; real 8 columns 5.7; 2 workgroups
+// per CU 4.2 / 5.15, 8 per CU 4.6 / 5.3.
+#ifndef RB_GEMV_COL_U
+#define RB_GEMV_COL_U 4
+#endif
+#ifndef RB_GEMV_COL_U_CPLX
+#define RB_GEMV_COL_U_CPLX 4
+#endif
+#ifndef RB_GEMV_COL_BLOCKS_PER_CU
+#define RB_GEMV_COL_BLOCKS_PER_CU 4
+#endif
+
 template <int CAP, bool FIXED, bool VEC, class F, bool CPLX = false>
 __global__ __launch_bounds__(kThreads) void colaxpy_kernel(GemvArgs<F> a) {
-  constexpr int U = 4;  // columns per step
+  constexpr int U = CPLX ? RB_GEMV_COL_U_CPLX : RB_GEMV_COL_U;  // columns per step
   constexpr int TU = CPLX ? 2 * U : U;  // terms per output per step
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
@@ -512,7 +528,7 @@ bool launch_cols(GemvArgs<F>& a, hipStream_t s) {
   const DeviceInfo* d = device_info();
   if (!d) return false;
   const int64_t rowblocks = (a.nout + 127) / 128;
-  int64_t chunks = (4 * (int64_t)d->cus + rowblocks - 1) / rowblocks;
+  int64_t chunks = (RB_GEMV_COL_BLOCKS_PER_CU * (int64_t)d->cus + rowblocks - 1) / rowblocks;
   const int64_t max_chunks = (a.len + 255) / 256;  // keep >= 256 columns per chunk
   if (chunks > max_chunks) chunks = max_chunks;
   if (chunks < 1) chunks = 1;
