@@ -142,6 +142,30 @@ def test_gemm_binned_accumulate(gpu):
     assert np.array_equal(Cb.view(np.uint64), Cb_e.view(np.uint64))
 
 
+@pytest.mark.parametrize("order", ["C", "R"])
+@pytest.mark.parametrize("fold", [3, 4])
+def test_gemv_binned_accumulate(gpu, order, fold):
+    """binnedBLAS_dbdgemv accumulates into binned outputs (dbdgemv.ccog:57-162):
+    two calls with different alpha equal the oracle's two calls, for both
+    memory patterns (terms contiguous and lda apart) and an odd output count."""
+    import reproblas_amd as rb
+
+    M, N = 301, 700
+    lda = N if order == "R" else M
+    O = oracle()
+    A = dg.wide(M * N, 97)
+    x = dg.wide(max(M, N), 98)
+    for trans in ("N", "T"):
+        ny = M if trans == "N" else N
+        nx = N if trans == "N" else M
+        Yb = np.zeros(ny * 2 * fold)
+        Ye = Yb.copy()
+        for alpha in (1.0, -0.25):
+            rb.dbdgemv(fold, order, trans, M, N, alpha, A, lda, x[:nx].copy(), 1, Yb)
+            O.fn["dbdgemv"](fold, order.encode(), trans.encode(), M, N, alpha, vp(A), lda, vp(x), 1, vp(Ye), 1)
+        assert eq_arrays(Yb, Ye), (order, trans)
+
+
 @pytest.mark.slow
 def test_gemv_full_size(gpu):
     """Benchmark scale: 16384^2 column-major N and T, fold 3, beta in {0, 0.5}."""
