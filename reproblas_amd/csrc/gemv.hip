@@ -480,7 +480,8 @@ RB_UNROLL_CAP
 constexpr int kRowBlocksPerCU = RB_GEMV_ROW_BLOCKS_PER_CU;
 // 1: the row-dot walks rows in pairs (rowdot2_kernel), 0: one row per wave.
 // On MI355X, 16384^2 column-major T at fold 3: double 5.8 -> 6.4 TB/s with
-// pairs, float 5.9 -> 5.75 TB/s (pairs off).
+// pairs; float 5.9 TB/s without, 5.6-5.75 with pairs at 4, 8 or 16 floats per
+// lane (pairs off). 2 or 8 workgroups per CU: within noise of 4.
 #ifndef RB_GEMV_ROW_PAIR_D
 #define RB_GEMV_ROW_PAIR_D 1
 #endif
