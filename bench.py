@@ -54,6 +54,11 @@ def wide_gpu(n, seed, device):
     return torch.ldexp(m, e).mul_(s)
 
 
+# untimed calls before each HBM-bound extra configuration (a few ms: the
+# clocks settle before the timed calls)
+WARM_HBM = 20
+
+
 def time_launches(fn, steps, warmup, stream):
     """Average time per call of fn (which enqueues on `stream`), HIP events."""
     import torch
@@ -151,7 +156,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra configurations)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", choices=["dsum_2p20", "dsum_2p28", "dnrm2_2p28", "folds", "dgemv", "dgemm", "ragged_batch",
@@ -284,7 +289,7 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
                          ("dnrm2_2p28", lambda: L.reproblas_hip_rdnrm2_device(3, n, vp(x), 1, vp(out), sp))):
             if not want(name):
                 continue
-            t = time_launches(fn, steps, 2, stream)
+            t = time_launches(fn, steps, WARM_HBM, stream)
             gbs = 8 * n / t / 1e9
             ex[name] = {"ms": round(t * 1e3, 3), "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
             log(f"[bench] {name}: {t * 1e3:.3f} ms, {gbs:.1f} GB/s")
@@ -299,12 +304,12 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
         rec = {"ddot": {}, "dnrm2": {}}
         for fold in (2, 3, 4):
             t = time_launches(lambda: L.reproblas_hip_rddot_device(fold, n, vp(x), 1, vp(y), 1, vp(out), sp),
-                              steps, 2, stream)
+                              steps, WARM_HBM, stream)
             rec["ddot"][str(fold)] = {"ms": round(t * 1e3, 3), "GB/s": round(16 * n / t / 1e9, 1)}
-            t = time_launches(lambda: L.reproblas_hip_rdnrm2_device(fold, n, vp(x), 1, vp(out), sp), steps, 2, stream)
+            t = time_launches(lambda: L.reproblas_hip_rdnrm2_device(fold, n, vp(x), 1, vp(out), sp), steps, WARM_HBM, stream)
             rec["dnrm2"][str(fold)] = {"ms": round(t * 1e3, 3), "GB/s": round(8 * n / t / 1e9, 1)}
         h = n // 2
-        t = time_launches(lambda: L.reproblas_hip_rdsum_device(3, h, vp(x), 2, vp(out), sp), steps, 2, stream)
+        t = time_launches(lambda: L.reproblas_hip_rdsum_device(3, h, vp(x), 2, vp(out), sp), steps, WARM_HBM, stream)
         rec["dsum_inc2"] = {"terms": h, "ms": round(t * 1e3, 3), "GB/s_of_terms": round(8 * h / t / 1e9, 1),
                             "GB/s_of_span": round(16 * h / t / 1e9, 1)}
         ex["folds"] = ex["folds_2p28"] = rec
@@ -321,7 +326,7 @@ def extras(L, rb, dev, stream, sp, vp, args, only=None):
         for tr in ("N", "T"):
             fn = lambda: L.reproblas_hip_rdgemv_device(3, b"C", tr.encode(), m, m, 1.0, vp(A), m, vp(xv), 1, 0.5,
                                                        vp(yv), 1, sp)
-            t = time_launches(fn, steps, 2, stream)
+            t = time_launches(fn, steps, WARM_HBM, stream)
             byts = 8 * (m * m + m + 2 * m)
             ex[f"dgemv_{tr}_16384"] = {"ms": round(t * 1e3, 3), "GB/s": round(byts / t / 1e9, 1),
                                        "hbm_frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
