@@ -14,6 +14,15 @@
  * reproblas_hip_last_error()). Calls that share a stream share the library's
  * per-stream scratch, so calls on one stream are ordered and safe; calls on
  * different streams use different scratch.
+ *
+ * Threads: every entry point may be called from several host threads at
+ * once. The host side of calls on the same (device, stream) is serialised
+ * by a per-stream lock (held from staging through the final synchronize of
+ * a synchronous call), so concurrent calls never share a staging or scratch
+ * buffer in flight; calls on different streams do not contend. The error
+ * state, the stream of the synchronous calls and the gemm tiling are per
+ * host thread. A batch plan must not be launched concurrently with itself
+ * (its tables and partials are the plan's own).
  */
 #ifndef REPROBLAS_HIP_H_
 #define REPROBLAS_HIP_H_
@@ -37,8 +46,8 @@ int reproblas_hip_last_error(void);
 const char *reproblas_hip_last_error_string(void);
 void reproblas_hip_clear_error(void);
 
-/* Stream used by the synchronous reproBLAS_/binnedBLAS_ calls (like a BLAS
- * handle's stream). Default: the null stream. */
+/* Stream used by the synchronous reproBLAS_/binnedBLAS_ calls of the calling
+ * host thread (like a BLAS handle's stream). Default: the null stream. */
 void reproblas_hip_set_stream(void *stream);
 void *reproblas_hip_get_stream(void);
 
@@ -123,8 +132,9 @@ int reproblas_hip_rcgemm_device(int fold, char Order, char TransA, char TransB, 
                                 const float *alpha, const float *A, int64_t lda, const float *B, int64_t ldb,
                                 const float *beta, float *C, int64_t ldc, void *stream);
 
-/* gemm tiling (the autotuner's knob; every tiling gives identical bits).
- * Returns the number of tilings; set/get select by ordinal. */
+/* gemm tiling (the autotuner's knob; every tiling gives identical bits) of
+ * the calling host thread. Returns the number of tilings; set/get select by
+ * ordinal. */
 int reproblas_hip_gemm_tiling_count(void);
 int reproblas_hip_gemm_set_tiling(int tiling);
 int reproblas_hip_gemm_get_tiling(void);

@@ -71,6 +71,10 @@ _SIGS = {
     "binnedBLAS_dbdasum": (None, [_c_int, _c_int, _vp, _c_int, _vp]),
     "binnedBLAS_dbddot": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp]),
     "binnedBLAS_dbdssq": (_c_double, [_c_int, _c_int, _vp, _c_int, _c_double, _vp]),
+    "binnedBLAS_dmdsum": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int]),
+    "binnedBLAS_dmdasum": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int]),
+    "binnedBLAS_dmddot": (None, [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_int]),
+    "binnedBLAS_dmdssq": (_c_double, [_c_int, _c_int, _vp, _c_int, _c_double, _vp, _c_int, _vp, _c_int]),
     "binnedBLAS_dbdgemv": (None, [_c_int, _c_char, _c_char, _c_int, _c_int, _c_double, _vp, _c_int, _vp, _c_int,
                                   _vp, _c_int]),
     "binnedBLAS_dbdgemm": (None, [_c_int, _c_char, _c_char, _c_char, _c_int, _c_int, _c_int, _c_double, _vp, _c_int,

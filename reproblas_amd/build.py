@@ -17,12 +17,17 @@ CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libreproblas_hip.so")
 
+# A unit is a source file, or (source file, preprocessor definitions, object
+# name) for a file compiled more than once.
 SOURCES = [
     "runtime.hip",
     "host_binned.hip",
     "binned_vec.hip",
     "blas1_d.hip",
     "blas1_f.hip",
+    "blas1_seg.hip",
+    *[("blas1_inst.hip", {"RB_INST_TERM": str(t), "RB_INST_TYPE": ty}, f"blas1_{name}_{ty[0]}")
+      for t, name in enumerate(("sum", "asum", "dot", "ssq")) for ty in ("double", "float")],
     "complex_d.hip",
     "complex_f.hip",
     "complex_mat.hip",
@@ -51,6 +56,14 @@ FLAGS = [
 ]
 
 
+def _unit(u):
+    """(source, definitions, object path) of a SOURCES entry."""
+    if isinstance(u, str):
+        return u, {}, os.path.join(OBJDIR, os.path.splitext(u)[0] + ".o")
+    src, defs, name = u
+    return src, defs, os.path.join(OBJDIR, name + ".o")
+
+
 def _stale(src, obj):
     if not os.path.exists(obj):
         return True
@@ -60,11 +73,11 @@ def _stale(src, obj):
     return any(os.path.getmtime(d) > t for d in deps)
 
 
-def _compile(src, incremental=False):
-    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+def _compile(unit, incremental=False):
+    src, defs, obj = _unit(unit)
     if incremental and not _stale(src, obj):
         return obj, ""
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *[f"-D{k}={v}" for k, v in defs.items()], "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")
@@ -78,9 +91,12 @@ def build(jobs=None, verbose=False, incremental=False):
     every header (for development); the default recompiles everything."""
     os.makedirs(OBJDIR, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 1)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    units = [u for u in SOURCES if os.path.exists(os.path.join(CSRC, _unit(u)[0]))]
+    # longest units first, so the pool finishes together
+    heavy = ("blas1_inst", "complex_d", "complex_f", "gemm", "gemv")
+    units.sort(key=lambda u: not any(h in _unit(u)[2] for h in heavy))
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda f: _compile(f, incremental), srcs))
+        results = list(ex.map(lambda f: _compile(f, incremental), units))
     if verbose:
         for _, err in results:
             if err.strip():

@@ -35,6 +35,7 @@ F device_amax(int N, const F* X, int incX, const F* Y, int incY) {
   clear_error();
   if (N <= 0) return F(0);
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool dev = is_device_pointer(X) && (!Y || is_device_pointer(Y));
   const int64_t ax = incX < 0 ? -(int64_t)incX : incX, ay = incY < 0 ? -(int64_t)incY : incY;
   const F* dx = X;

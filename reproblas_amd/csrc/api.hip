@@ -146,13 +146,16 @@ void reproBLAS_rdgemv(const int fold, const char Order, const char TransA, const
   if (M <= 0 || N <= 0) return;
   if (!check_fold(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool nt = (TransA == 'n' || TransA == 'N');
   const int64_t lenx = nt ? N : M, leny = nt ? M : N;
-  bool sa, sx, sy;
-  const double* dA = stage_matrix(A, matrix_span(Order, M, N, lda), kSlotStageX, s, &sa);
-  const double* dX = stage_vector(X, lenx, incX, kSlotStageY, s, &sx);
+  bool sa = false, sx = false, sy;
+  // alpha == 0: no product is formed and A, X are never read (rdgemv.c:35-87)
+  const bool prod = alpha != 0.0;
+  const double* dA = prod ? stage_matrix(A, matrix_span(Order, M, N, lda), kSlotStageX, s, &sa) : nullptr;
+  const double* dX = prod ? stage_vector(X, lenx, incX, kSlotStageY, s, &sx) : nullptr;
   double* dY = (double*)stage_vector(Y, leny, incY, kSlotStageZ, s, &sy);
-  if (!dA || !dX || !dY) return;
+  if ((prod && (!dA || !dX)) || !dY) return;
   if (!rdgemv_device(fold, Order, TransA, M, N, alpha, dA, lda, dX, incX, beta, dY, incY, s)) return;
   if (sy && !unstage_vector(Y, dY, leny, incY, s)) return;
   RB_HIP(hipStreamSynchronize(s));
@@ -171,13 +174,19 @@ void reproBLAS_rdgemm(const int fold, const char Order, const char TransA, const
   if (M <= 0 || N <= 0) return;
   if (!check_fold(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool na = (TransA == 'n' || TransA == 'N'), nb = (TransB == 'n' || TransB == 'N');
-  bool sa, sb, sc;
-  const double* dA = stage_matrix(A, matrix_span(Order, na ? M : K, na ? K : M, lda), kSlotStageX, s, &sa);
-  const double* dB = stage_matrix(B, matrix_span(Order, nb ? K : N, nb ? N : K, ldb), kSlotStageY, s, &sb);
+  bool sa = false, sb = false, sc;
+  // alpha == 0 or K <= 0: no product is formed and A, B are never read
+  // (dbdgemm.ccog:77-80)
+  const bool prod = K > 0 && alpha != 0.0;
+  const double* dA =
+      prod ? stage_matrix(A, matrix_span(Order, na ? M : K, na ? K : M, lda), kSlotStageX, s, &sa) : nullptr;
+  const double* dB =
+      prod ? stage_matrix(B, matrix_span(Order, nb ? K : N, nb ? N : K, ldb), kSlotStageY, s, &sb) : nullptr;
   const int64_t cspan = matrix_span(Order, M, N, ldc);
   double* dC = (double*)stage_matrix(C, cspan, kSlotStageZ, s, &sc);
-  if ((K > 0 && (!dA || !dB)) || !dC) return;
+  if ((prod && (!dA || !dB)) || !dC) return;
   if (!rdgemm_device(fold, Order, TransA, TransB, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, s)) return;
   if (sc && !RB_HIP(hipMemcpyAsync(C, dC, sizeof(double) * (size_t)cspan, hipMemcpyDeviceToHost, s))) return;
   RB_HIP(hipStreamSynchronize(s));
@@ -196,6 +205,7 @@ void binnedBLAS_dbdgemv(const int fold, const char Order, const char TransA, con
   if (M <= 0 || N <= 0 || alpha == 0.0) return;
   if (!check_fold(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool nt = (TransA == 'n' || TransA == 'N');
   const int64_t lenx = nt ? N : M, leny = nt ? M : N;
   bool sa, sx;
@@ -226,6 +236,7 @@ void binnedBLAS_dbdgemm(const int fold, const char Order, const char TransA, con
   if (M <= 0 || N <= 0 || K <= 0 || alpha == 0.0) return;
   if (!check_fold(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool na = (TransA == 'n' || TransA == 'N'), nb = (TransB == 'n' || TransB == 'N');
   bool sa, sb;
   const double* dA = stage_matrix(A, matrix_span(Order, na ? M : K, na ? K : M, lda), kSlotStageX, s, &sa);
@@ -247,6 +258,7 @@ int reproblas_hip_rdgemv_device(int fold, char Order, char TransA, int64_t M, in
                                 const double* A, int64_t lda, const double* X, int64_t incX, double beta, double* Y,
                                 int64_t incY, void* stream) {
   clear_error();
+  RB_STREAM_GUARD(stream);
   if (M <= 0 || N <= 0) return kOk;
   if (!check_fold(fold)) return last_error();
   if (!require_device_accessible(Y, "Y") ||
@@ -261,6 +273,7 @@ int reproblas_hip_rdgemm_device(int fold, char Order, char TransA, char TransB, 
                                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
                                 double beta, double* C, int64_t ldc, void* stream) {
   clear_error();
+  RB_STREAM_GUARD(stream);
   if (M <= 0 || N <= 0) return kOk;
   if (!check_fold(fold)) return last_error();
   if (!require_device_accessible(C, "C") ||
@@ -384,13 +397,16 @@ void reproBLAS_rsgemv(const int fold, const char Order, const char TransA, const
   if (M <= 0 || N <= 0) return;
   if (!check_fold<float>(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool nt = (TransA == 'n' || TransA == 'N');
   const int64_t lenx = nt ? N : M, leny = nt ? M : N;
-  bool sa, sx, sy;
-  const float* dA = stage_matrix(A, matrix_span(Order, M, N, lda), kSlotStageX, s, &sa);
-  const float* dX = stage_vector(X, lenx, incX, kSlotStageY, s, &sx);
+  bool sa = false, sx = false, sy;
+  // alpha == 0: no product is formed and A, X are never read (rdgemv.c:35-87)
+  const bool prod = alpha != 0.0f;
+  const float* dA = prod ? stage_matrix(A, matrix_span(Order, M, N, lda), kSlotStageX, s, &sa) : nullptr;
+  const float* dX = prod ? stage_vector(X, lenx, incX, kSlotStageY, s, &sx) : nullptr;
   float* dY = (float*)stage_vector(Y, leny, incY, kSlotStageZ, s, &sy);
-  if (!dA || !dX || !dY) return;
+  if ((prod && (!dA || !dX)) || !dY) return;
   if (!rsgemv_device(fold, Order, TransA, M, N, alpha, dA, lda, dX, incX, beta, dY, incY, s)) return;
   if (sy && !unstage_vector(Y, dY, leny, incY, s)) return;
   RB_HIP(hipStreamSynchronize(s));
@@ -408,13 +424,19 @@ void reproBLAS_rsgemm(const int fold, const char Order, const char TransA, const
   if (M <= 0 || N <= 0) return;
   if (!check_fold<float>(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool na = (TransA == 'n' || TransA == 'N'), nb = (TransB == 'n' || TransB == 'N');
-  bool sa, sb, sc;
-  const float* dA = stage_matrix(A, matrix_span(Order, na ? M : K, na ? K : M, lda), kSlotStageX, s, &sa);
-  const float* dB = stage_matrix(B, matrix_span(Order, nb ? K : N, nb ? N : K, ldb), kSlotStageY, s, &sb);
+  bool sa = false, sb = false, sc;
+  // alpha == 0 or K <= 0: no product is formed and A, B are never read
+  // (dbdgemm.ccog:77-80)
+  const bool prod = K > 0 && alpha != 0.0f;
+  const float* dA =
+      prod ? stage_matrix(A, matrix_span(Order, na ? M : K, na ? K : M, lda), kSlotStageX, s, &sa) : nullptr;
+  const float* dB =
+      prod ? stage_matrix(B, matrix_span(Order, nb ? K : N, nb ? N : K, ldb), kSlotStageY, s, &sb) : nullptr;
   const int64_t cspan = matrix_span(Order, M, N, ldc);
   float* dC = (float*)stage_matrix(C, cspan, kSlotStageZ, s, &sc);
-  if ((K > 0 && (!dA || !dB)) || !dC) return;
+  if ((prod && (!dA || !dB)) || !dC) return;
   if (!rsgemm_device(fold, Order, TransA, TransB, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, s)) return;
   if (sc && !RB_HIP(hipMemcpyAsync(C, dC, sizeof(float) * (size_t)cspan, hipMemcpyDeviceToHost, s))) return;
   RB_HIP(hipStreamSynchronize(s));
@@ -433,6 +455,7 @@ void binnedBLAS_sbsgemv(const int fold, const char Order, const char TransA, con
   if (M <= 0 || N <= 0 || alpha == 0.0f) return;
   if (!check_fold<float>(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool nt = (TransA == 'n' || TransA == 'N');
   const int64_t lenx = nt ? N : M, leny = nt ? M : N;
   bool sa, sx;
@@ -463,6 +486,7 @@ void binnedBLAS_sbsgemm(const int fold, const char Order, const char TransA, con
   if (M <= 0 || N <= 0 || K <= 0 || alpha == 0.0f) return;
   if (!check_fold<float>(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool na = (TransA == 'n' || TransA == 'N'), nb = (TransB == 'n' || TransB == 'N');
   bool sa, sb;
   const float* dA = stage_matrix(A, matrix_span(Order, na ? M : K, na ? K : M, lda), kSlotStageX, s, &sa);
@@ -484,6 +508,7 @@ int reproblas_hip_rsgemv_device(int fold, char Order, char TransA, int64_t M, in
                                 int64_t lda, const float* X, int64_t incX, float beta, float* Y, int64_t incY,
                                 void* stream) {
   clear_error();
+  RB_STREAM_GUARD(stream);
   if (M <= 0 || N <= 0) return kOk;
   if (!check_fold<float>(fold)) return last_error();
   if (!require_device_accessible(Y, "Y") ||
@@ -498,6 +523,7 @@ int reproblas_hip_rsgemm_device(int fold, char Order, char TransA, char TransB, 
                                 float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
                                 float* C, int64_t ldc, void* stream) {
   clear_error();
+  RB_STREAM_GUARD(stream);
   if (M <= 0 || N <= 0) return kOk;
   if (!check_fold<float>(fold)) return last_error();
   if (!require_device_accessible(C, "C") ||

@@ -48,6 +48,7 @@ void run_zblas1(int fold, ZTerm term, int64_t n, const F* X, int64_t incX, const
   if (!check_fold<F>(fold)) return;
   if (n < 0) n = 0;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool dot = (term == kZDotU || term == kZDotC);
   bool sx = false, sy = false;
   const F* dX = stage_cvector(X, n, incX, kSlotStageX, s, &sx);
@@ -111,6 +112,7 @@ template <class F>
 int device_zblas1(int fold, ZTerm term, int64_t N, const F* X, int64_t incX, const F* Y, int64_t incY, F* result,
                   void* stream) {
   clear_error();
+  RB_STREAM_GUARD(stream);
   if (!check_fold<F>(fold)) return last_error();
   if (N < 0) N = 0;
   const bool dot = (term == kZDotU || term == kZDotC);
@@ -177,6 +179,7 @@ void run_rzgemv(int fold, char Order, char TransA, int M, int N, const F* alpha,
   if (M <= 0 || N <= 0) return;
   if (!check_fold<F>(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const int64_t lenx = notrans(TransA) ? N : M, leny = notrans(TransA) ? M : N;
   bool sa, sx, sy;
   const F* dA = stage_matrix(A, 2 * matrix_span(Order, M, N, lda), kSlotStageX, s, &sa);
@@ -196,6 +199,7 @@ void run_rzgemm(int fold, char Order, char TransA, char TransB, int M, int N, in
   if (M <= 0 || N <= 0) return;
   if (!check_fold<F>(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool na = notrans(TransA), nb = notrans(TransB);
   bool sa, sb, sc;
   const F* dA = stage_matrix(A, 2 * matrix_span(Order, na ? M : K, na ? K : M, lda), kSlotStageX, s, &sa);
@@ -216,6 +220,7 @@ void run_zbzgemv(int fold, char Order, char TransA, int M, int N, const F* alpha
   if (M <= 0 || N <= 0 || (alpha[0] == F(0) && alpha[1] == F(0))) return;
   if (!check_fold<F>(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const int64_t lenx = notrans(TransA) ? N : M, leny = notrans(TransA) ? M : N;
   bool sa, sx;
   const F* dA = stage_matrix(A, 2 * matrix_span(Order, M, N, lda), kSlotStageX, s, &sa);
@@ -245,6 +250,7 @@ void run_zbzgemm(int fold, char Order, char TransA, char TransB, int M, int N, i
   if (M <= 0 || N <= 0 || K <= 0 || (alpha[0] == F(0) && alpha[1] == F(0))) return;
   if (!check_fold<F>(fold)) return;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   const bool na = notrans(TransA), nb = notrans(TransB);
   bool sa, sb;
   const F* dA = stage_matrix(A, 2 * matrix_span(Order, na ? M : K, na ? K : M, lda), kSlotStageX, s, &sa);
@@ -266,6 +272,7 @@ template <class F>
 int device_rzgemv(int fold, char Order, char TransA, int64_t M, int64_t N, const F* alpha, const F* A, int64_t lda,
                   const F* X, int64_t incX, const F* beta, F* Y, int64_t incY, void* stream) {
   clear_error();
+  RB_STREAM_GUARD(stream);
   if (M <= 0 || N <= 0) return kOk;
   if (!check_fold<F>(fold)) return last_error();
   const bool az = alpha[0] == F(0) && alpha[1] == F(0);
@@ -281,6 +288,7 @@ template <class F>
 int device_rzgemm(int fold, char Order, char TransA, char TransB, int64_t M, int64_t N, int64_t K, const F* alpha,
                   const F* A, int64_t lda, const F* B, int64_t ldb, const F* beta, F* C, int64_t ldc, void* stream) {
   clear_error();
+  RB_STREAM_GUARD(stream);
   if (M <= 0 || N <= 0) return kOk;
   if (!check_fold<F>(fold)) return last_error();
   const bool az = alpha[0] == F(0) && alpha[1] == F(0);

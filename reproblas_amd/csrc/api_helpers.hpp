@@ -79,6 +79,7 @@ F run_blas1(int fold, Term term, int64_t n, const F* X, int64_t incX, const F* Y
   if (!check_fold<F>(fold)) return (F)kNaN;
   if (n < 0) n = 0;
   hipStream_t s = current_stream();
+  RB_STREAM_GUARD(s);
   bool sx = false, sy = false;
   const F* dX = stage_vector(X, n, incX, kSlotStageX, s, &sx);
   const F* dY = (term == kDot) ? stage_vector(Yv, n, incY, kSlotStageY, s, &sy) : nullptr;
@@ -142,6 +143,7 @@ template <class F>
 int device_blas1(int fold, Term term, int64_t N, const F* X, int64_t incX, const F* Y, int64_t incY,
                         const F* sin, const F* scale_in_dev, F* sout, F* scale_out, F* result, void* stream) {
   clear_error();
+  RB_STREAM_GUARD(stream);
   if (!check_fold<F>(fold)) return last_error();
   if (N < 0) N = 0;
   // the kernels dereference these directly: refuse host memory up front

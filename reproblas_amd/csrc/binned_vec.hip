@@ -83,6 +83,32 @@ RB_UNROLL_CAP
   v[i * incv] = convert(fold, p, c);
 }
 
+// y(i, o) = conv(binned(beta * y(i, o))) in place for an inner x outer array,
+// element (i, o) at y[i*inc_i + o*inc_o]: the whole result of
+// reproBLAS_rdgemv / rdgemm when no product is added (alpha == 0 or K == 0;
+// src/reproBLAS/rdgemm.c:56-103, rdgemv.c:48-84). beta == 0 gives +0.
+template <int CAP, bool FIXED, class F>
+__global__ __launch_bounds__(256) void beta_only_kernel(int fold_rt, int64_t n, int64_t inner, F* y, int64_t inc_i,
+                                                        int64_t inc_o, F beta) {
+  const int fold = FIXED ? CAP : fold_rt;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    F* e = y + (t % inner) * inc_i + (t / inner) * inc_o;
+    F p[CAP], c[CAP];
+    set_zero(fold, p, c);
+    if (beta != F(0)) {
+      const F x = (beta == F(1)) ? *e : *e * beta;
+      update(fold, x, p, c);
+      if (nan_or_inf(x) || nan_or_inf(p[0])) {
+        p[0] += x;
+      } else {
+        deposit(fold, dmindex0(p[0]), x, p);
+      }
+      renorm(fold, p, c);
+    }
+    *e = convert(fold, p, c);
+  }
+}
+
 // Complex binned values: both components, interleaved (4*fold per value).
 template <int CAP, bool FIXED, class F>
 __global__ __launch_bounds__(256) void zaddv_kernel(int fold_rt, int64_t n, const F* X, int64_t incX, F* Y,
@@ -194,6 +220,41 @@ bool conv_from_binned(int fold, int64_t n, const float* Yb, float* y, int64_t in
 
 bool conv_to_double(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s) {
   return conv_from_binned(fold, n, Yb, y, incy, s);
+}
+
+namespace {
+template <class F>
+bool beta_only_impl(int fold, int64_t inner, int64_t outer, F* y, int64_t inc_i, int64_t inc_o, F beta,
+                    hipStream_t s) {
+  if (!fold_ok<F>(fold)) return false;
+  if (inner <= 0 || outer <= 0) return true;
+  const int64_t n = inner * outer;
+  const unsigned g = grid_for(n) < 65536u ? grid_for(n) : 65536u;  // grid-stride beyond
+#define RB_BETA_LAUNCH(CAP, FIXED) \
+  beta_only_kernel<CAP, FIXED, F><<<g, 256, 0, s>>>(fold, n, inner, y, inc_i, inc_o, beta)
+  switch (fold) {
+    case 2: RB_BETA_LAUNCH(2, true); break;
+    case 3: RB_BETA_LAUNCH(3, true); break;
+    case 4: RB_BETA_LAUNCH(4, true); break;
+    default:
+      if (fold <= 8)
+        RB_BETA_LAUNCH(8, false);
+      else
+        RB_BETA_LAUNCH(Fmt<F>::max_fold, false);
+      break;
+  }
+#undef RB_BETA_LAUNCH
+  return RB_HIP(hipGetLastError());
+}
+}  // namespace
+
+bool beta_only(int fold, int64_t inner, int64_t outer, double* y, int64_t inc_i, int64_t inc_o, double beta,
+               hipStream_t s) {
+  return beta_only_impl<double>(fold, inner, outer, y, inc_i, inc_o, beta, s);
+}
+bool beta_only(int fold, int64_t inner, int64_t outer, float* y, int64_t inc_i, int64_t inc_o, float beta,
+               hipStream_t s) {
+  return beta_only_impl<float>(fold, inner, outer, y, inc_i, inc_o, beta, s);
 }
 
 }  // namespace rb

@@ -1,6 +1,6 @@
-// blas1_d.hip -- the double instances of blas1_impl.hpp (blas1, the ragged
-// segment kernels); blas1_f.hip holds the float ones, so the two compile in
-// parallel.
+// blas1_d.hip -- the double BLAS1 entry of blas1_impl.hpp (argument set-up
+// and grid; the kernel instances are in the blas1_inst.hip units, the ragged
+// segment kernels in blas1_seg.hip).
 #include "blas1_impl.hpp"
 
 namespace rb {
@@ -14,31 +14,10 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
                             result, s);
 }
 
-bool segments(int fold, Term term, const double* x, const double* y, const SegChunk* chunks, int64_t nchunks,
-              double* partials, const int64_t* first_chunk, int64_t nseg, double* results, int grid, hipStream_t s) {
-  if (!fold_supported(fold)) {
-    set_error(kBadFold, "fold %d is outside the supported range [2, 52]", fold);
-    return false;
-  }
-  SegArgs a;
-  a.x = x;
-  a.y = y;
-  a.chunks = chunks;
-  a.nchunks = nchunks;
-  a.partials = partials;
-  a.first_chunk = first_chunk;
-  a.nseg = nseg;
-  a.results = results;
-  a.fold = fold;
-  if (grid < 1) grid = 1;
-  switch (fold) {
-    case 2: return launch_seg_term<2, true>(term, a, grid, s);
-    case 3: return launch_seg_term<3, true>(term, a, grid, s);
-    case 4: return launch_seg_term<4, true>(term, a, grid, s);
-    default:
-      if (fold <= 8) return launch_seg_term<8, false>(term, a, grid, s);
-      return launch_seg_term<kMaxFold, false>(term, a, grid, s);
-  }
+bool blas1_scratch(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
+                   double* result, void* scratch, size_t scratch_bytes, unsigned* ticket, hipStream_t s) {
+  return blas1_impl<double>(fold, term, n, x, incx, y, incy, nullptr, 0.0, nullptr, nullptr, nullptr, result, s,
+                            scratch, scratch_bytes, ticket);
 }
 
 }  // namespace rb

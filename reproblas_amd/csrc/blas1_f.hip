@@ -1,4 +1,5 @@
-// blas1_f.hip -- the float instances of blas1_impl.hpp.
+// blas1_f.hip -- the float BLAS1 entry of blas1_impl.hpp (argument set-up and
+// grid; the kernel instances are in the blas1_inst.hip units).
 #include "blas1_impl.hpp"
 
 namespace rb {

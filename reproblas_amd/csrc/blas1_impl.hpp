@@ -29,11 +29,6 @@
 
 namespace rb {
 
-namespace {
-
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-
 template <class F>
 struct Blas1Args {
   int64_t n;
@@ -42,6 +37,7 @@ struct Blas1Args {
   const F* y;
   int64_t incy;
   F* partials;    // gridDim.x records of (2*fold+1) elements
+  int64_t* aux;   // sum / dot: gridDim.x records {first infinite term, big} (PoisonInfo)
   unsigned* ticket;
   const F* state_in;
   F scale_in;
@@ -52,21 +48,161 @@ struct Blas1Args {
   int fold;
 };
 
+// One launch of the BLAS1 kernel of a term on prepared arguments (mode: 0
+// unit stride 16-byte aligned, 1 unit stride, 2 strided). Each is compiled
+// in a unit of its own (blas1_inst.hip built once per term and type), so the
+// kernel instances compile in parallel.
+bool blas1_launch_sum(int mode, const Blas1Args<double>& a, int blocks, hipStream_t s);
+bool blas1_launch_asum(int mode, const Blas1Args<double>& a, int blocks, hipStream_t s);
+bool blas1_launch_dot(int mode, const Blas1Args<double>& a, int blocks, hipStream_t s);
+bool blas1_launch_ssq(int mode, const Blas1Args<double>& a, int blocks, hipStream_t s);
+bool blas1_launch_sum(int mode, const Blas1Args<float>& a, int blocks, hipStream_t s);
+bool blas1_launch_asum(int mode, const Blas1Args<float>& a, int blocks, hipStream_t s);
+bool blas1_launch_dot(int mode, const Blas1Args<float>& a, int blocks, hipStream_t s);
+bool blas1_launch_ssq(int mode, const Blas1Args<float>& a, int blocks, hipStream_t s);
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+
+// Block length of the reference's driver for a term kind: binnedBLAS_dmdsum
+// and dmddot walk blocks of 2048 and 1024 doubles (dmdsum.ccog:36,
+// dmddot.ccog:36), smssum and smsdot blocks of 512 floats (smssum.ccog:37,
+// smsdot.ccog:36). Each renormalises before a block once `endurance`
+// deposits would be exceeded (dmdsum.ccog:81-84): before every block after
+// the first when the block is the endurance, before every second one when
+// it is half of it.
+template <int T, class F>
+constexpr int ref_block() {
+  return sizeof(F) == sizeof(double) ? (T == kDot ? 1024 : 2048) : 512;
+}
+
+// The binned sum of the terms [lo, hi) by the 4 waves of one workgroup;
+// thread 0 ends with the merged value (renormalised). Every thread calls it.
+template <int CAP, int T, int U, int MODE, class F>
+__device__ void wg_range_sum(const Blas1Args<F>& a, int fold, int64_t lo, int64_t hi, int wave, int lane,
+                             F (*lds)[2 * CAP + 1], State<CAP, F>& out) {
+  const Span<F> sp = make_span<U>(a.x + lo * a.incx, a.incx, T == kDot ? a.y + lo * a.incy : nullptr, a.incy,
+                                  hi - lo);
+  State<CAP, F> st;
+  wave_accumulate<CAP, T, U, MODE>(sp, fold, wave, kWaves, lane, st, nullptr);
+  __syncthreads();
+  if (lane == 0) store_state(lds[wave], fold, st);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    State<CAP, F> o;
+    load_state(lds[0], fold, out);
+    for (int w = 1; w < kWaves; ++w) {
+      load_state(lds[w], fold, o);
+      add(fold, out.p, out.c, o.p, o.c);
+    }
+  }
+}
+
+// The reference's infinity path (dmdsum.ccog:66-78, dmddot.ccog:68-81):
+// at the first block holding an infinite term it adds that block's terms
+// one by one, in order, to its top primary priY[0] -- the state of all
+// earlier blocks, renormalised at the last renorm the cadence made and not
+// since -- and afterwards only combines classes. The result is NaN when that
+// sequential sum overflows to the infinity opposite to the block's first
+// infinite term before reaching it, else that infinity. Called by every
+// thread of the last workgroup when the merged class is one infinity and a
+// finite term (or the caller's value) of index 0 occurs; pstar is the first
+// infinite term. Thread 0 returns the reference's priY[0] after that block
+// (which is then the result: later blocks add only same-sign infinities or
+// finite terms to it).
+template <int CAP, int T, int U, int MODE, class F>
+__device__ F replay_infinity_path(const Blas1Args<F>& a, int fold, int64_t pstar, int wave, int lane,
+                                  F (*lds)[2 * CAP + 1], F* seq) {
+  constexpr int64_t NB = ref_block<T, F>();
+  constexpr int64_t kPer = Fmt<F>::endurance / NB;  // blocks per renorm
+  const int64_t bs = pstar / NB;                    // the first block with an infinite term
+  F p0 = F(0);
+  if (bs > 0) {
+    // blocks [r, bs) were deposited since the last renorm (before block r)
+    const int64_t r = ((bs - 1) / kPer) * kPer;
+    State<CAP, F> R, S;
+    if (r >= 1) wg_range_sum<CAP, T, U, MODE>(a, fold, 0, r * NB, wave, lane, lds, R);
+    wg_range_sum<CAP, T, U, MODE>(a, fold, r * NB, bs * NB, wave, lane, lds, S);
+    if (threadIdx.x == 0) {
+      if (r >= 1) {
+        if (a.state_in) {
+          State<CAP, F> y;
+RB_UNROLL_CAP
+          for (int j = 0; j < CAP; ++j)
+            if (j < fold) { y.p[j] = a.state_in[j]; y.c[j] = a.state_in[fold + j]; }
+          renorm(fold, y.p, y.c);
+          add(fold, R.p, R.c, y.p, y.c);
+        }
+      } else if (a.state_in) {
+        // no renorm yet: the caller's value as given (dmdsum.ccog:81-84 skips block 0)
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) { R.p[j] = a.state_in[j]; R.c[j] = a.state_in[fold + j]; }
+      } else {
+        set_zero(fold, R.p, R.c);
+      }
+      // the updates of blocks [r, bs) leave the index at min(I_R, I_S)
+      // (dmdupdate.c:24-56); the top slot then holds R's content plus the
+      // top-bin slices of those blocks, exact (dmddeposit.c:35-70 within the
+      // endurance)
+      const int is = dmindex(S.p[0]);
+      int ip = is;
+      if (R.p[0] == F(0)) {
+        set_bins(fold, ip, R.p, R.c);
+      } else {
+        const int ir = dmindex(R.p[0]);
+        ip = ir < is ? ir : is;
+        raise_index(fold, ir - ip, ip, R.p, R.c);
+      }
+      raise_index(fold, is - ip, ip, S.p, S.c);
+      p0 = (R.p[0] + (S.p[0] - bin<F>(ip))) + S.c[0] * carry_unit<F>(ip);
+    }
+  } else if (threadIdx.x == 0 && a.state_in) {
+    p0 = a.state_in[0];
+  }
+  // the terms of block bs up to the first infinity, staged in order
+  const int64_t b0 = bs * NB;
+  for (int64_t e = b0 + threadIdx.x; e <= pstar; e += kThreads) {
+    const F xv = a.x[e * a.incx];
+    seq[e - b0] = plain_term<T>(xv, T == kDot ? a.y[e * a.incy] : F(0));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int64_t j = 0; j <= pstar - b0; ++j) p0 = p0 + seq[j];
+  }
+  return p0;
+}
+
 template <int CAP, bool FIXED, int T, int U, int MODE, class F>
 __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
+  constexpr bool kInfPath = (T == kSum || T == kDot);
   const int fold = FIXED ? CAP : a.fold;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar)
   const int64_t wave_gid = (int64_t)blockIdx.x * kWaves + wave;
   const int64_t nwaves = (int64_t)gridDim.x * kWaves;
-  const Span sp = make_span<U>(a.x, a.incx, a.y, a.incy, a.n);
+  const Span<F> sp = make_span<U>(a.x, a.incx, a.y, a.incy, a.n);
   State<CAP, F> s;
-  wave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s);
+  PoisonInfo pi;
+  // nrm2: every wave squares at no less than the caller's scale
+  const F scale0 = (T == kSSq && a.state_in) ? (a.scale_in_dev ? *a.scale_in_dev : a.scale_in) : F(0);
+  wave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s, kInfPath ? &pi : nullptr, scale0);
 
   // Workgroup merge through LDS, then one partial per workgroup.
   __shared__ F lds[kWaves][2 * CAP + 1];
+  __shared__ int64_t lds_fi[kWaves];
+  __shared__ int lds_big[kWaves];
   __shared__ bool last;
-  if (lane == 0) store_state(lds[wave], fold, s);
+  if (lane == 0) {
+    store_state(lds[wave], fold, s);
+    if (kInfPath) {
+      lds_fi[wave] = pi.first_inf;
+      lds_big[wave] = pi.big;
+    }
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     State<CAP, F> acc, o;
@@ -76,6 +212,16 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
       merge<CAP, T>(fold, acc, o);
     }
     store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, acc);
+    if (kInfPath) {
+      int64_t fi = lds_fi[0];
+      int big = lds_big[0];
+      for (int w = 1; w < kWaves; ++w) {
+        fi = lds_fi[w] < fi ? lds_fi[w] : fi;
+        big |= lds_big[w];
+      }
+      a.aux[2 * (int64_t)blockIdx.x] = fi;
+      a.aux[2 * (int64_t)blockIdx.x + 1] = big;
+    }
     __threadfence();
     const unsigned ticket = atomicAdd(a.ticket, 1u);
     last = (ticket == gridDim.x - 1);
@@ -88,6 +234,8 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
   __threadfence();
   State<CAP, F> acc;
   bool have = false;
+  int64_t fi = INT64_MAX;
+  bool big = false;
   for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
     State<CAP, F> o;
     const F* src = a.partials + (int64_t)b * (2 * fold + 1);
@@ -98,6 +246,11 @@ RB_UNROLL_CAP
         o.c[j] = __builtin_nontemporal_load(src + fold + j);
       }
     o.scl = __builtin_nontemporal_load(src + 2 * fold);
+    if (kInfPath) {
+      const int64_t f = a.aux[2 * (int64_t)b];
+      fi = f < fi ? f : fi;
+      big |= a.aux[2 * (int64_t)b + 1] != 0;
+    }
     if (!have) {
       acc = o;
       have = true;
@@ -110,8 +263,20 @@ RB_UNROLL_CAP
     acc.scl = 0.0;
   }
   wave_merge_general<CAP, T>(fold, acc);
-  if (lane == 0) store_state(lds[wave], fold, acc);
+  if (kInfPath) {
+    fi = wave_min_i64(fi);
+    big = __any(big);
+  }
+  if (lane == 0) {
+    store_state(lds[wave], fold, acc);
+    if (kInfPath) {
+      lds_fi[wave] = fi;
+      lds_big[wave] = big;
+    }
+  }
   __syncthreads();
+  State<CAP, F> y;
+  __shared__ int64_t replay_at;  // first infinite term, or -1: no replay
   if (threadIdx.x == 0) {
     State<CAP, F> o;
     load_state(lds[0], fold, acc);
@@ -120,7 +285,6 @@ RB_UNROLL_CAP
       merge<CAP, T>(fold, acc, o);
     }
     // caller's value, renormalised first (the reference ends with a renorm)
-    State<CAP, F> y;
     if (a.state_in) {
 RB_UNROLL_CAP
       for (int j = 0; j < CAP; ++j)
@@ -133,13 +297,43 @@ RB_UNROLL_CAP
       set_zero(fold, y.p, y.c);
     }
     y.scl = a.scale_in_dev ? *a.scale_in_dev : a.scale_in;
-    if (a.n == 0) {
-      // nothing deposited: the caller's value only (the reference's final renorm)
-    } else if (y.p[0] == 0.0) {
-      y = acc;
-    } else {
+    replay_at = -1;
+    if (kInfPath) {
+      int64_t f = lds_fi[0];
+      bool bg = lds_big[0] != 0;
+      for (int w = 1; w < kWaves; ++w) {
+        f = lds_fi[w] < f ? lds_fi[w] : f;
+        bg |= lds_big[w] != 0;
+      }
+      // one infinity, a finite caller value, and magnitudes that can overflow
+      const bool one_inf = nan_or_inf(acc.p[0]) && acc.p[0] == acc.p[0];
+      // (f < n always holds for an infinite class; checked so no replay can
+      // ever index past the vector)
+      if (a.n > 0 && one_inf && f >= 0 && f < a.n && !nan_or_inf(y.p[0]) && (bg || dmindex0(y.p[0])))
+        replay_at = f;
+    }
+  }
+  bool replayed = false;
+  if constexpr (kInfPath) {
+    __syncthreads();
+    const int64_t pstar = replay_at;
+    if (pstar >= 0) {
+      __shared__ F seq[ref_block<T, F>()];
+      // the reference's top primary after its first infinite block: it
+      // already holds the caller's value, and later blocks leave it as is
+      const F top = replay_infinity_path<CAP, T, U, MODE>(a, fold, pstar, wave, lane, lds, seq);
+      if (threadIdx.x == 0) y.p[0] = top;
+      replayed = true;
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (a.n > 0 && !replayed) {
+      // An empty caller value still carries its scale for nrm2: the reference
+      // squares every term at max(scaleY, dscale(amax)) (dmdssq.ccog:62-101),
+      // which the scaled merge reproduces by rescaling acc up to scaleY.
       merge<CAP, T>(fold, y, acc);
     }
+    // n == 0: nothing deposited, the caller's value only (the reference's final renorm)
     renorm(fold, y.p, y.c);
     F scale = y.scl;
     if (T == kSSq && nan_or_inf(y.p[0])) scale = F(1);  // dscale(1.0), dmdssq.ccog tail
@@ -222,13 +416,17 @@ constexpr int u_dot() { return sizeof(F) == sizeof(double) ? kUDot : RB_BLAS1_U_
 template <class F>
 constexpr int blocks_per_cu() { return sizeof(F) == sizeof(double) ? RB_BLAS1_BLOCKS_PER_CU : RB_BLAS1_BLOCKS_PER_CU_S; }
 
-template <int CAP, bool FIXED, class F>
-bool launch_term(Term term, int mode, const Blas1Args<F>& a, int blocks, hipStream_t s) {
-  switch (term) {
-    case kSum: return launch_mode<CAP, FIXED, kSum, u_one<F>()>(mode, a, blocks, s);
-    case kASum: return launch_mode<CAP, FIXED, kASum, u_one<F>()>(mode, a, blocks, s);
-    case kDot: return launch_mode<CAP, FIXED, kDot, u_dot<F>()>(mode, a, blocks, s);
-    default: return launch_mode<CAP, FIXED, kSSq, u_one<F>()>(mode, a, blocks, s);
+// The kernel instance of term T for the arguments' fold.
+template <int T, class F>
+bool launch_fold_term(int mode, const Blas1Args<F>& a, int blocks, hipStream_t s) {
+  constexpr int U = (T == kDot) ? u_dot<F>() : u_one<F>();
+  switch (a.fold) {
+    case 2: return launch_mode<2, true, T, U>(mode, a, blocks, s);
+    case 3: return launch_mode<3, true, T, U>(mode, a, blocks, s);
+    case 4: return launch_mode<4, true, T, U>(mode, a, blocks, s);
+    default:
+      if (a.fold <= 8) return launch_mode<8, false, T, U>(mode, a, blocks, s);
+      return launch_mode<Fmt<F>::max_fold, false, T, U>(mode, a, blocks, s);
   }
 }
 
@@ -318,10 +516,13 @@ bool launch_seg_term(Term term, const SegArgs& a, int grid, hipStream_t s) {
 }  // namespace
 
 namespace {
+// ext_scratch (or null for the library workspace): device memory of
+// ext_bytes for the partial and infinity records, and a zero ticket counter
+// that the launch leaves at zero.
 template <class F>
 bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const F* y, int64_t incy,
                 const F* state_in, F scale_in, const F* scale_in_dev, F* state_out, F* scale_out, F* result,
-                hipStream_t s) {
+                hipStream_t s, void* ext_scratch = nullptr, size_t ext_bytes = 0, unsigned* ext_ticket = nullptr) {
   if (fold < 2 || fold > Fmt<F>::max_fold) {
     set_error(kBadFold, "fold %d is outside the supported range [2, %d]", fold, Fmt<F>::max_fold);
     return false;
@@ -359,16 +560,32 @@ bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const 
   const int64_t cap = (int64_t)dev->cus * (mode == 2 ? kBlocksPerCUStrided : blocks_per_cu<F>());
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  a.partials = (F*)workspace(s, kSlotPartials, sizeof(F) * (size_t)blocks * (2 * fold + 1));
-  a.ticket = tickets(s, 1);
+  // partial records, then (8-byte aligned) the PoisonInfo records
+  const size_t rec_bytes = (sizeof(F) * (size_t)blocks * (2 * fold + 1) + 7) & ~(size_t)7;
+  const size_t need = rec_bytes + sizeof(int64_t) * 2 * (size_t)blocks;
+  char* ws;
+  if (ext_scratch) {
+    if (need > ext_bytes) {
+      set_error(kBadArgument,
+                "this reduction needs %zu bytes of scratch (%lld workgroup records at fold %d); the fixed scratch "
+                "holds %zu",
+                need, (long long)blocks, fold, ext_bytes);
+      return false;
+    }
+    ws = (char*)ext_scratch;
+    a.ticket = ext_ticket;
+  } else {
+    ws = (char*)workspace(s, kSlotPartials, need);
+    a.ticket = tickets(s, 1);
+  }
+  a.partials = (F*)ws;
+  a.aux = ws ? (int64_t*)(ws + rec_bytes) : nullptr;
   if (!a.partials || !a.ticket) return false;
-  switch (fold) {
-    case 2: return launch_term<2, true>(term, mode, a, (int)blocks, s);
-    case 3: return launch_term<3, true>(term, mode, a, (int)blocks, s);
-    case 4: return launch_term<4, true>(term, mode, a, (int)blocks, s);
-    default:
-      if (fold <= 8) return launch_term<8, false>(term, mode, a, (int)blocks, s);
-      return launch_term<Fmt<F>::max_fold, false>(term, mode, a, (int)blocks, s);
+  switch (term) {
+    case kSum: return blas1_launch_sum(mode, a, (int)blocks, s);
+    case kASum: return blas1_launch_asum(mode, a, (int)blocks, s);
+    case kDot: return blas1_launch_dot(mode, a, (int)blocks, s);
+    default: return blas1_launch_ssq(mode, a, (int)blocks, s);
   }
 }
 }  // namespace

@@ -222,10 +222,12 @@ __device__ __forceinline__ void deposit_ztile(int fold, const F (&xr)[U], const 
 // One wave reduces the tiles t0, t0 + tstep, ... into both channels.
 template <int CAP, int T, int U, int MODE, class F>
 __device__ __forceinline__ void zwave_accumulate(const ZSpan<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                                 State<CAP, F> (&s)[2]) {
+                                                 State<CAP, F> (&s)[2], F scale0 = F(0)) {
   WaveRun<F> w[2];
-  wave_start(fold, w[0], s[0]);
-  wave_start(fold, w[1], s[1]);
+  // nrm2: both channels square at no less than the caller's scale
+  // (dmzssq.ccog, as dmdssq.ccog:88-99)
+  wave_start(fold, w[0], s[0], scale0);
+  wave_start(fold, w[1], s[1], scale0);
   F axr[U], axi[U], ayr[U], ayi[U], bxr[U], bxi[U], byr[U], byi[U];
   int64_t t = t0;
   if (t < a.tiles) load_ztile<T, U, MODE>(a, t, lane, axr, axi, ayr, ayi);
@@ -278,7 +280,8 @@ __global__ __launch_bounds__(kThreads) void zblas1_kernel(ZArgs<F> a) {
   const int64_t nwaves = (int64_t)gridDim.x * kWaves;
   const ZSpan<F> sp = make_zspan<U>(a.x, a.incx, a.y, a.incy, a.n);
   State<CAP, F> s[2];
-  zwave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s);
+  const F scale0 = (T == kZSSq && a.state_in) ? (a.scale_in_dev ? *a.scale_in_dev : a.scale_in) : F(0);
+  zwave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s, scale0);
 
   __shared__ F lds[kWaves][2][2 * CAP + 1];
   __shared__ bool last;
@@ -387,12 +390,9 @@ RB_UNROLL_CAP
       set_zero(fold, y.p, y.c);
     }
     y.scl = a.scale_in_dev ? *a.scale_in_dev : a.scale_in;
-    if (a.n == 0) {
-    } else if (y.p[0] == F(0)) {
-      y = v;
-    } else {
-      merge<CAP, KM>(fold, y, v);
-    }
+    // an empty caller value keeps its scale (dmzssq squares at
+    // max(scaleY, dscale(amax))): always the scaled merge
+    if (a.n > 0) merge<CAP, KM>(fold, y, v);
     renorm(fold, y.p, y.c);
     F scale = y.scl;
     if (T == kZSSq && nan_or_inf(y.p[0])) scale = F(1);

@@ -370,7 +370,9 @@ bool launch_batch_tm(bool ai, bool bk, const GemmDesc* d, const GemmTile* t, int
   return RB_HIP(hipGetLastError());
 }
 
-int g_tiling = 1;  // 0: 4x4 outputs per lane, 1: 4x2, 2: 2x2
+// Tiling of the calling host thread (0: 4x4 outputs per lane, 1: 4x2, 2: 2x2);
+// every tiling gives the same bits, so threads may differ freely.
+thread_local int g_tiling = 1;
 
 // Tiling actually used for a fold (folds without a dedicated instance use 2x2).
 int effective_tiling(int tiling, int fold) { return (fold >= 2 && fold <= 4) ? tiling : 2; }
@@ -465,16 +467,11 @@ bool rgemm_impl(int fold, char order, char transa, char transb, int64_t M, int64
                 int64_t lda, const F* B, int64_t ldb, F beta, F* C, int64_t ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return true;  // rdgemm.c:50-52
   if (K <= 0 || alpha == F(0)) {
-    // no products: C still goes through binned(beta*C) and back
+    // no products (dbdgemm.ccog:77-80 adds nothing): C still goes through
+    // binned(beta*C) and back, one in-place launch over the whole matrix
     const bool rm = (order == 'r' || order == 'R');
     const int64_t outer = rm ? M : N, inner = rm ? N : M;
-    F* tmp = (F*)workspace(s, kSlotBinnedOut, sizeof(F) * (size_t)inner * 2 * fold);
-    if (!tmp) return false;
-    for (int64_t o = 0; o < outer; ++o) {
-      if (!conv_to_binned(fold, inner, C + o * ldc, 1, beta, tmp, s)) return false;
-      if (!conv_from_binned(fold, inner, tmp, C + o * ldc, 1, s)) return false;
-    }
-    return true;
+    return beta_only(fold, inner, outer, C, 1, ldc, beta, s);
   }
   return gemm_common(make_desc(fold, order, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                                (F*)nullptr, 0), s);

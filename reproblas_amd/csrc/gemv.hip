@@ -126,7 +126,7 @@ __device__ __forceinline__ Span<F> row_span(const GemvArgs<F>& a, const F* xs, i
 template <int CAP, int MODE, class F>
 __device__ __forceinline__ void finish_row(const GemvArgs<F>& a, const F* xs, int fold, int64_t o, int lane,
                                            const WaveRun<F>& w, State<CAP, F>& s) {
-  wave_finish<CAP, kDot, u_row<F>(), MODE>(row_span(a, xs, o), fold, 0, 1, lane, w, s);
+  wave_finish<CAP, kDot, u_row<F>(), MODE>(row_span(a, xs, o), fold, 0, 1, lane, w, s, nullptr);
   if (lane == 0) finish_output(a, fold, o, s.p, s.c);
 }
 
@@ -634,9 +634,7 @@ bool rgemv_impl(int fold, char order, char trans, int64_t M, int64_t N, F alpha,
   const int64_t nout = notrans ? M : N;
   if (alpha == F(0)) {
     // dbdgemv adds nothing; y still goes through binned(beta*y) and back
-    F* tmp = (F*)workspace(s, kSlotBinnedOut, sizeof(F) * (size_t)nout * 2 * fold);
-    if (!tmp) return false;
-    return conv_to_binned(fold, nout, y, incy, beta, tmp, s) && conv_from_binned(fold, nout, tmp, y, incy, s);
+    return beta_only(fold, nout, 1, y, incy, 0, beta, s);
   }
   return gemv_common<F>(fold, order, trans, M, N, alpha, A, lda, x, incx, nullptr, 0, y, incy, beta, s);
 }

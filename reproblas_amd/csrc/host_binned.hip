@@ -140,6 +140,7 @@ void binned_dbdbaddv(const int fold, const int N, const double_binned* X, const 
   const bool dx = is_device_pointer(X), dy = is_device_pointer(Y);
   if (dx && dy) {
     hipStream_t s = current_stream();
+    RB_STREAM_GUARD(s);
     if (dbdbaddv(fold, N, X, incX, Y, incY, s)) RB_HIP(hipStreamSynchronize(s));
     return;
   }
@@ -328,6 +329,7 @@ void binned_sbsbaddv(const int fold, const int N, const float_binned* X, const i
   const bool dx = is_device_pointer(X), dy = is_device_pointer(Y);
   if (dx && dy) {
     hipStream_t s = current_stream();
+    RB_STREAM_GUARD(s);
     if (sbsbaddv(fold, N, X, incX, Y, incY, s)) RB_HIP(hipStreamSynchronize(s));
     return;
   }

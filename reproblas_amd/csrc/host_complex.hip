@@ -75,6 +75,7 @@ void binned_zbzbaddv(const int fold, const int N, const double_complex_binned* X
   if (rb::is_device_pointer(X) && rb::is_device_pointer(Y)) {
     rb::clear_error();
     hipStream_t s = rb::current_stream();
+    RB_STREAM_GUARD(s);
     if (rb::zaddv(fold, N, X, incX, Y, incY, s)) RB_HIP(hipStreamSynchronize(s));
     return;
   }
@@ -215,6 +216,7 @@ void binned_cbcbaddv(const int fold, const int N, const float_complex_binned* X,
   if (rb::is_device_pointer(X) && rb::is_device_pointer(Y)) {
     rb::clear_error();
     hipStream_t s = rb::current_stream();
+    RB_STREAM_GUARD(s);
     if (rb::zaddv(fold, N, X, incX, Y, incY, s)) RB_HIP(hipStreamSynchronize(s));
     return;
   }

@@ -24,6 +24,13 @@ enum Term : int { kSum = 0, kASum = 1, kDot = 2, kSSq = 3 };
 bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
            const double* state_in, double scale_in, const double* scale_in_dev, double* state_out,
            double* scale_out, double* result, hipStream_t s);
+// The converted double result of a BLAS1 reduction with the caller's scratch
+// instead of the library workspace: scratch holds scratch_bytes of device
+// memory for the workgroup records (the launch fails with kBadArgument when
+// the grid needs more), ticket a device counter that is zero on entry and
+// that the launch leaves at zero. Allocates nothing.
+bool blas1_scratch(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
+                   double* result, void* scratch, size_t scratch_bytes, unsigned* ticket, hipStream_t s);
 // The same for float operands and float_binned values (folds 2..21).
 bool blas1_s(int fold, Term term, int64_t n, const float* x, int64_t incx, const float* y, int64_t incy,
              const float* state_in, float scale_in, const float* scale_in_dev, float* state_out, float* scale_out,
@@ -69,6 +76,13 @@ bool conv_to_binned(int fold, int64_t n, const float* y, int64_t incy, float bet
 bool conv_from_binned(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s);
 bool conv_from_binned(int fold, int64_t n, const float* Yb, float* y, int64_t incy, hipStream_t s);
 bool conv_to_double(int fold, int64_t n, const double* Yb, double* y, int64_t incy, hipStream_t s);
+// beta_only: y(i, o) = conv(binned(beta*y(i, o))) in place for i < inner,
+// o < outer, element (i, o) at y[i*inc_i + o*inc_o] (+0 when beta == 0): the
+// rdgemv / rdgemm result when no product is added. One launch, no scratch.
+bool beta_only(int fold, int64_t inner, int64_t outer, double* y, int64_t inc_i, int64_t inc_o, double beta,
+               hipStream_t s);
+bool beta_only(int fold, int64_t inner, int64_t outer, float* y, int64_t inc_i, int64_t inc_o, float beta,
+               hipStream_t s);
 
 // Binned gemv on device operands (reference binnedBLAS_dbdgemv semantics,
 // column- or row-major): Yb[i*incY] (+)= sum_j fl(op(A)_ij * fl(alpha*x_j)).

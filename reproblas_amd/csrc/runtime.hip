@@ -15,7 +15,7 @@ namespace {
 thread_local int g_err = kOk;
 thread_local char g_msg[512] = "";
 std::mutex g_mu;
-hipStream_t g_stream = nullptr;
+thread_local hipStream_t g_stream = nullptr;
 
 struct Buf {
   void* ptr = nullptr;
@@ -30,7 +30,25 @@ std::map<std::pair<int, hipStream_t>, Buf>& ticket_bufs() {
   return *m;
 }
 DeviceInfo g_info[64];
+
+// One recursive mutex per (device, stream), created on first use and kept
+// (a stream handle that is destroyed and reused maps to the same mutex).
+std::recursive_mutex* stream_mutex(hipStream_t s) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    dev = -1;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  static auto* m = new std::map<std::pair<int, hipStream_t>, std::recursive_mutex*>();
+  std::recursive_mutex*& p = (*m)[std::make_pair(dev, s)];
+  if (!p) p = new std::recursive_mutex();
+  return p;
+}
 }  // namespace
+
+StreamGuard::StreamGuard(hipStream_t s) : mu_(stream_mutex(s)) { static_cast<std::recursive_mutex*>(mu_)->lock(); }
+StreamGuard::~StreamGuard() { static_cast<std::recursive_mutex*>(mu_)->unlock(); }
 
 void set_error(int code, const char* fmt, ...) {
   g_err = code;

@@ -29,9 +29,29 @@ const char* last_error_message();
 bool hip_ok(hipError_t e, const char* what);
 #define RB_HIP(call) ::rb::hip_ok((call), #call)
 
-// Stream used by the synchronous entry points (default: the null stream).
+// Stream used by the synchronous entry points of the calling host thread
+// (default: the null stream). Each thread has its own setting.
 hipStream_t current_stream();
 void set_current_stream(hipStream_t s);
+
+// Serialises the host side of library calls that share a (device, stream):
+// every entry point holds it from its first workspace() / tickets() fetch
+// through its last enqueue (synchronous calls: through their final
+// synchronize), so concurrent host threads never stage into, or regrow and
+// free, a buffer another thread's call is still using. Recursive: an entry
+// point may call another under the same guard. Calls on different streams
+// do not contend.
+class StreamGuard {
+ public:
+  explicit StreamGuard(hipStream_t s);
+  ~StreamGuard();
+  StreamGuard(const StreamGuard&) = delete;
+  StreamGuard& operator=(const StreamGuard&) = delete;
+
+ private:
+  void* mu_;
+};
+#define RB_STREAM_GUARD(s) ::rb::StreamGuard rb_stream_guard_((hipStream_t)(s))
 
 struct DeviceInfo {
   int device = -1;
@@ -44,12 +64,14 @@ const DeviceInfo* device_info();
 
 // Device memory of at least `bytes` owned by the library, one slot per
 // (device, stream, slot id), grown on demand and kept until process exit.
-// Calls that share a stream are ordered by that stream, so sharing a slot
-// between them is safe.
+// Calls that share a stream are ordered by that stream on the device, and
+// their host sides are serialised by StreamGuard, so sharing a slot between
+// them is safe. Callers must hold StreamGuard(s).
 void* workspace(hipStream_t s, int slot, size_t bytes);
 
 // Zero-initialised device counters (one per slot) used by the single-pass
-// grid reductions; every kernel that takes one leaves it at zero.
+// grid reductions; every kernel that takes one leaves it at zero. Callers
+// must hold StreamGuard(s).
 unsigned* tickets(hipStream_t s, int count);
 
 // Pointer kind of `p`: true when a kernel may dereference it directly.

@@ -149,6 +149,18 @@ __device__ __forceinline__ void load_rows16(const F* base, int lane, F (&v)[U]) 
   }
 }
 
+// Element index (within the span) of term k of this lane in tile t, in the
+// layout load_tile uses: MODE 0 full tiles hold 16-byte vectors of V
+// consecutive elements, 64 lanes interleaved per vector row; every other
+// tile holds element k*64 + lane.
+template <int U, int MODE, class F>
+__device__ __forceinline__ int64_t tile_element(const Span<F>& a, int64_t t, int k, int lane) {
+  constexpr int V = 16 / (int)sizeof(F);
+  const int64_t base = t * (64 * U);
+  if (MODE == 0 && t < a.full_tiles) return base + (int64_t)V * ((k / V) * 64 + lane) + (k % V);
+  return base + (int64_t)k * 64 + lane;
+}
+
 // Loads the U raw terms of tile t for this lane. MODE 0: unit stride with
 // 16-byte vector loads (operands 16-byte aligned); MODE 1: unit stride,
 // element loads; MODE 2: arbitrary stride. Out-of-range terms read as 0,
@@ -241,22 +253,54 @@ RB_UNROLL_CAP
   }
 }
 
+// What a wave reports about infinities besides its binned value, for the
+// replay of the reference's infinity path (sum and dot only):
+//   first_inf -- element index of the wave's first infinite term (INT64_MAX
+//                when it has none),
+//   big       -- the wave saw a finite term of index 0 (|t| >= 2^984 for
+//                double), the only terms whose sequential sum can overflow.
+struct PoisonInfo {
+  int64_t first_inf;
+  bool big;
+};
+
+__device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const int64_t o = __shfl_xor((long long)v, m, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
 // Class of the terms of the tiles this wave walked: NaN if any term is NaN or
 // both infinities occur, else the infinity that occurs. Terms as the
-// reference's infinity path forms them.
+// reference's infinity path forms them. With `info`, also the wave's first
+// infinite term and whether a finite term of index 0 occurs.
 template <int K, int U, int MODE, class F>
-__device__ F wave_term_class(const Span<F>& a, int64_t first, int64_t step, int lane) {
-  bool nan = false, pinf = false, ninf = false;
+__device__ F wave_term_class(const Span<F>& a, int64_t first, int64_t step, int lane, PoisonInfo* info) {
+  bool nan = false, pinf = false, ninf = false, big = false;
+  int64_t fi = INT64_MAX;
   for (int64_t t = first; t < a.tiles; t += step) {
     F vx[U], vy[U];
     load_tile<K, U, MODE>(a, t, lane, vx, vy);
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const F v = plain_term<K>(vx[k], K == kDot ? vy[k] : F(0));
+      const bool inf = fabs(v) == (F)__builtin_inf();
       nan |= (v != v);
       pinf |= (v == (F)__builtin_inf());
       ninf |= (v == -(F)__builtin_inf());
+      if (info) {
+        const int64_t e = tile_element<U, MODE>(a, t, k, lane);
+        if (inf && e < fi) fi = e;
+        big |= !inf && fabs(v) >= admit_limit<F>(1);
+      }
     }
+  }
+  if (info) {
+    info->first_inf = wave_min_i64(fi);
+    info->big = __any(big);
   }
   const bool any_nan = __any(nan), any_p = __any(pinf), any_n = __any(ninf);
   if (any_nan || (any_p && any_n) || !(any_p || any_n)) return (F)__builtin_nan("");
@@ -341,14 +385,16 @@ __device__ __forceinline__ void deposit_tile(int fold, const F (&cx)[U], const F
 
 // Fresh wave state: every wave starts at the last index, the reference's
 // representation of an empty sum (dmdupdate of 0.0 on a zero value), so every
-// tile deposits.
+// tile deposits. nrm2 starts at the caller's scale (0 for none): the
+// reference squares every term at max(scaleY, dscale(amax so far))
+// (dmdssq.ccog:88-99), so no term is ever squared at a smaller scale.
 template <int CAP, class F>
-__device__ __forceinline__ void wave_start(int fold, WaveRun<F>& w, State<CAP, F>& s) {
+__device__ __forceinline__ void wave_start(int fold, WaveRun<F>& w, State<CAP, F>& s, F scale0 = F(0)) {
   w.idx = Fmt<F>::max_index;
   set_bins(fold, w.idx, s.p, s.c);
   w.lim = admit_limit<F>(w.idx);
-  w.scl = F(0);
-  w.inv = F(0);
+  w.scl = scale0;
+  w.inv = scale0 > F(0) ? F(1) / scale0 : F(0);  // exact: scales are powers of two
   w.dep = 0;
 }
 
@@ -379,21 +425,24 @@ RB_UNROLL_CAP
 
 template <int CAP, int K, int U, int MODE, class F>
 __device__ __forceinline__ void wave_finish(const Span<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                            const WaveRun<F>& w, State<CAP, F>& s) {
+                                            const WaveRun<F>& w, State<CAP, F>& s, PoisonInfo* info) {
   wave_butterfly(fold, w, s);
   if (nan_or_inf(uniform_value(s.p[0]))) {
-    s.p[0] = wave_term_class<K, U, MODE>(a, t0, tstep, lane);
+    s.p[0] = wave_term_class<K, U, MODE>(a, t0, tstep, lane, info);
+  } else if (info) {
+    info->first_inf = INT64_MAX;
+    info->big = (w.idx == 0);
   }
 }
 
 // One wave reduces the tiles t0, t0 + tstep, ... of a span into `s` (every
 // lane ends with the wave's value; a poisoned value is replaced by the class
-// of the terms).
+// of the terms). `info` (or null) receives what the infinity-path replay needs.
 template <int CAP, int K, int U, int MODE, class F>
 __device__ __forceinline__ void wave_accumulate(const Span<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                                State<CAP, F>& s) {
+                                                State<CAP, F>& s, PoisonInfo* info = nullptr, F scale0 = F(0)) {
   WaveRun<F> w;
-  wave_start(fold, w, s);
+  wave_start(fold, w, s, scale0);
 
   // Two tile buffers in turn: the loads of the next tile are in flight while
   // the current one is deposited, and no buffer is ever copied.
@@ -411,7 +460,7 @@ __device__ __forceinline__ void wave_accumulate(const Span<F>& a, int fold, int6
     deposit_tile<CAP, K, U>(fold, bx, by, w, s);
     t = tn;
   }
-  wave_finish<CAP, K, U, MODE>(a, fold, t0, tstep, lane, w, s);
+  wave_finish<CAP, K, U, MODE>(a, fold, t0, tstep, lane, w, s, info);
 }
 
 }  // namespace

@@ -1,0 +1,99 @@
+"""GPU parity in the regions where the reference's result depends on its
+block schedule (the oracle is pinned there against the compiled reference
+and tests/golden/corners.npz by tests/test_oracle_corners.py):
+
+* the infinity path of rdsum / rddot / rssum / rsdot (dmdsum.ccog:66-78,
+  dmddot.ccog:68-81): a sequential overflow before the first infinity of the
+  reference's first infinite block makes the result NaN;
+* nrm2 over values spanning 2^-1000..2^1000 with the largest values in late
+  blocks (dmdssq.ccog:60-101), at every fold.
+
+Bit for bit against the oracle (NaN compared by class), host and device
+operands, unit and non-unit strides.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import test_oracle_corners as tc
+from cpu_libs import oracle, oracle_s
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_run(rb, op, dt, fold, x, y, inc, device):
+    import torch
+
+    if device is not None:
+        x = torch.from_numpy(x).to(device)
+        y = torch.from_numpy(y).to(device) if y is not None else None
+    if dt == np.float64:
+        return rb.rdsum(x, fold, inc) if op == "sum" else rb.rddot(x, y, fold, inc, inc)
+    return rb.rssum(x, fold, inc) if op == "sum" else rb.rsdot(x, y, fold, inc, inc)
+
+
+@pytest.mark.parametrize("key,op,dt,kind,n,seed,inc", tc.CASES, ids=[c[0] for c in tc.CASES])
+def test_infinity_path(gpu, key, op, dt, kind, n, seed, inc):
+    import reproblas_amd as rb
+
+    x, y = tc.make_input(op, dt, kind, n, seed, inc)
+    orc = oracle() if dt == np.float64 else oracle_s()
+    eq = tc.same if dt == np.float64 else tc.same32
+    for fold in (2, 3, 4):
+        e = tc.run(orc, op, dt, fold, x, y, inc)
+        for device in (None, gpu):
+            r = gpu_run(rb, op, dt, fold, x, y, inc, device)
+            assert eq(r, e), (key, fold, "device" if device is not None else "host", r, e)
+
+
+def test_infinity_path_accumulate_into_top_state(gpu):
+    """binnedBLAS_dbdsum into a caller value at index 0, host and device Y."""
+    import torch
+
+    import reproblas_amd as rb
+    import datagen as dg
+
+    O = oracle()
+    for fold in (2, 3, 4):
+        Y0 = np.zeros(2 * fold)
+        O.dbdsum(fold, np.array([1.6e308, 1.0]), Y0)
+        x = dg.uniform(5000, 9)
+        x[3000], x[3001] = 1.0e308, -np.inf
+        Ye = Y0.copy()
+        O.dbdsum(fold, x, Ye)
+        Yh = Y0.copy()
+        rb.dbdsum(Yh, x, fold)
+        assert tc.same(Yh[0], Ye[0]), fold
+        Yd = torch.from_numpy(Y0.copy()).to(gpu)
+        rb.dbdsum(Yd, torch.from_numpy(x).to(gpu), fold)
+        assert tc.same(Yd[0].item(), Ye[0]), fold
+
+
+@pytest.mark.parametrize("seed,fold", tc.nrm2_range_cases())
+def test_nrm2_wide_range(gpu, seed, fold):
+    """rdnrm2 and the dbdssq state over a 2^2000 dynamic range whose largest
+    values arrive in late blocks (the reference rescales block by block)."""
+    import torch
+
+    import reproblas_amd as rb
+
+    x = tc.nrm2_range_input(seed)
+    O = oracle()
+    e = O.rdnrm2(fold, x)
+    tx = torch.from_numpy(x).to(gpu)
+    assert tc.same(rb.rdnrm2(tx, fold), e), (seed, fold)
+    assert tc.same(rb.rdnrm2(x, fold), e), (seed, fold)
+    Ye = np.zeros(2 * fold)
+    se = O.dbdssq(fold, x, 0.0, Ye)
+    Yg = np.zeros(2 * fold)
+    sg = rb.dbdssq(Yg, x, 0.0, fold)
+    assert tc.same(sg, se)
+    # every slot inside the bin table; past it (folds >= 28 here) the
+    # reference's own content depends on how its calls split the terms
+    # (test_oracle_corners.test_ssq_slots_beyond_bin_table_depend_on_the_split)
+    keep = tc.table_slots(fold, Ye, O)
+    if fold <= 27:
+        assert len(keep) == 2 * fold
+    assert np.array_equal(Yg[keep].view(np.uint64), Ye[keep].view(np.uint64)), (seed, fold)
+    assert tc.same(O.ddbconv(fold, Yg), O.ddbconv(fold, Ye))

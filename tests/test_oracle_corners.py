@@ -1,0 +1,247 @@
+"""Pins the oracle in the two regions where the reference's result depends on
+its block schedule rather than on the multiset of terms:
+
+* the infinity path of dmdsum / dmddot / smssum / smsdot
+  (src/binnedBLAS/dmdsum.ccog:66-78, dmddot.ccog:68-81): at its first block
+  holding an infinite term the reference adds that block's terms one by one
+  to its top primary, so a sequential overflow before the infinity turns the
+  result into NaN;
+* the scaled sum of squares (dmdssq.ccog:60-101) over values spanning most
+  of the exponent range with the largest values in late blocks, at every
+  fold, and into an empty accumulator that carries the caller's scale.
+
+Each case is checked against both builds of the compiled reference
+(oracle/_ref) when present, and always against tests/golden/corners.npz,
+which the compiled reference wrote (tests/golden/make_golden_corners.py).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import datagen as dg
+from cpu_libs import oracle, oracle_s, reference, reference_s
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "corners.npz")
+
+
+def bits64(v):
+    return np.array([v], dtype=np.float64).view(np.uint64)[0]
+
+
+def same(a, b):
+    if np.isnan(a) and np.isnan(b):
+        return True
+    return bits64(a) == bits64(b)
+
+
+def same32(a, b):
+    a, b = np.float32(a), np.float32(b)
+    if np.isnan(a) and np.isnan(b):
+        return True
+    return a.view(np.uint32) == b.view(np.uint32)
+
+
+def _golden():
+    if not os.path.exists(GOLDEN):
+        pytest.skip("tests/golden/corners.npz not written")
+    return np.load(GOLDEN)
+
+
+def corner_cases():
+    """(key, op, dtype, kind, n, seed, inc) of every infinity-path case; the
+    inputs are regenerated from datagen with these parameters."""
+    out = []
+    for dt, tag in ((np.float64, "d"), (np.float32, "s")):
+        for op in ("sum", "dot"):
+            for kind in dg.INF_CORNERS:
+                seeds = range(8) if kind == "fuzz" else (1, 2)
+                for seed in seeds:
+                    for inc in (1, 2):
+                        n = 9000 if dt == np.float64 else 3000
+                        out.append((f"inf_{tag}_{op}_{kind}_{seed}_{inc}", op, dt, kind, n, seed, inc))
+    return out
+
+
+def make_input(op, dt, kind, n, seed, inc):
+    x, y = dg.inf_corner(kind, n, seed, op, dt)
+    if inc != 1:
+        xs = np.zeros(n * inc, dt)
+        xs[::inc] = x
+        x = xs
+        if y is not None:
+            ys = np.zeros(n * inc, dt)
+            ys[::inc] = y
+            y = ys
+    return x, y
+
+
+def run(lib, op, dt, fold, x, y, inc):
+    if dt == np.float64:
+        if op == "sum":
+            return lib.rdsum(fold, x, inc)
+        return lib.rddot(fold, x, y, inc, inc)
+    n = len(x) // inc
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)
+    if op == "sum":
+        return lib.fn["rssum"](fold, n, p(x), inc)
+    return lib.fn["rsdot"](fold, n, p(x), inc, p(y), inc)
+
+
+CASES = corner_cases()
+
+
+@pytest.mark.parametrize("key,op,dt,kind,n,seed,inc", CASES, ids=[c[0] for c in CASES])
+def test_infinity_path_matches_reference(key, op, dt, kind, n, seed, inc):
+    x, y = make_input(op, dt, kind, n, seed, inc)
+    orc = oracle() if dt == np.float64 else oracle_s()
+    refs = [reference(k) if dt == np.float64 else reference_s(k) for k in ("avx", "sisd")]
+    eq = same if dt == np.float64 else same32
+    G = _golden()
+    for fold in (2, 3, 4):
+        e = run(orc, op, dt, fold, x, y, inc)
+        for R in refs:
+            if R is not None:
+                assert eq(e, run(R, op, dt, fold, x, y, inc)), (key, fold)
+        assert eq(e, G[f"{key}_f{fold}"][0]), (key, fold, e, G[f"{key}_f{fold}"][0])
+
+
+def test_infinity_path_cases_reach_both_outcomes():
+    """The corner set is not vacuous: it holds results that are NaN only
+    because of a sequential overflow (the class of the terms is one
+    infinity) and results that stay the infinity."""
+    G = _golden()
+    nan_one_sign = inf_kept = 0
+    for key, op, dt, kind, n, seed, inc in CASES:
+        x, y = make_input(op, dt, kind, n, seed, inc)
+        with np.errstate(over="ignore", invalid="ignore"):
+            t = x[::inc] if op == "sum" else x[::inc] * y[::inc]
+        one_sign = not np.isnan(t).any() and not (np.isposinf(t).any() and np.isneginf(t).any())
+        r = G[f"{key}_f3"][0]
+        if one_sign and np.isnan(r):
+            nan_one_sign += 1
+        if np.isinf(r):
+            inf_kept += 1
+    assert nan_one_sign >= 20 and inf_kept >= 20, (nan_one_sign, inf_kept)
+
+
+def test_infinity_path_accumulate_into_top_state():
+    """binnedBLAS_dbdsum into a caller value already at index 0: the
+    reference's top primary starts from the caller's, so one big term before
+    an infinity overflows it (dmdsum.ccog:66-78)."""
+    O = oracle()
+    G = _golden()
+    for fold in (2, 3, 4):
+        Y = np.zeros(2 * fold)
+        O.dbdsum(fold, np.array([1.6e308, 1.0]), Y)
+        x = dg.uniform(5000, 9)
+        x[3000], x[3001] = 1.0e308, -np.inf
+        Ye = Y.copy()
+        O.dbdsum(fold, x, Ye)
+        for k in ("avx", "sisd"):
+            R = reference(k)
+            if R is not None:
+                Yr = Y.copy()
+                R.dbdsum(fold, x, Yr)
+                assert same(Yr[0], Ye[0]), fold
+        assert same(Ye[0], G[f"acc_top_f{fold}"][0]), fold
+        assert np.isnan(Ye[0])
+
+
+def nrm2_range_cases():
+    out = []
+    for seed in range(6):
+        for fold in (2, 3, 4, 8, 13, 20, 26, 27, 30, 40, 52):
+            out.append((seed, fold))
+    return out
+
+
+def nrm2_range_input(seed):
+    """Values spanning 2^-1000 .. 2^1000 with the largest in late blocks, so
+    the reference's running scale rises block by block."""
+    g = dg.rng(900 + seed)
+    n = 6 * 2048 + 333
+    x = dg.uniform(n, 900 + seed) * np.exp2(g.integers(-1000, -600, n).astype(float))
+    for b, e in ((1, -300), (2, 0), (3, 400), (4, 800), (5, 1000)):
+        x[b * 2048 + int(g.integers(0, 2048))] = 1.25 * 2.0 ** e
+    return x
+
+
+@pytest.mark.parametrize("seed,fold", nrm2_range_cases())
+def test_nrm2_wide_range_matches_reference(seed, fold):
+    x = nrm2_range_input(seed)
+    O = oracle()
+    G = _golden()
+    e = O.rdnrm2(fold, x)
+    Y = np.zeros(2 * fold)
+    s = O.dbdssq(fold, x, 0.0, Y)
+    for k in ("avx", "sisd"):
+        R = reference(k)
+        if R is not None:
+            assert same(R.rdnrm2(fold, x), e), (k, fold)
+            if k == "sisd":
+                Yr = np.zeros(2 * fold)
+                assert same(R.dbdssq(fold, x, 0.0, Yr), s)
+                assert np.array_equal(Yr.view(np.uint64), Y.view(np.uint64)), (fold, Yr, Y)
+    assert same(e, G[f"nrm2_{seed}_f{fold}"][0])
+    assert np.array_equal(Y.view(np.uint64), G[f"ssq_{seed}_f{fold}"].view(np.uint64))
+
+
+@pytest.mark.parametrize("fold", [2, 3, 4, 7])
+def test_ssq_empty_value_with_scale_matches_reference(fold):
+    """dmdssq.ccog:62-101 with Y == 0 and scaleY above dscale(max|x|): the
+    scale stays scaleY (for x = [1, 2, 3] and scaleY = dscale(1e60) the
+    reference returns scaleY, not dscale(3))."""
+    O = oracle()
+    G = _golden()
+    for j, (x, sy) in enumerate(((np.array([1.0, 2.0, 3.0]), O.dscale(1e60)),
+                                 (dg.wide(3000, 530), O.dscale(1e-3)),
+                                 (np.zeros(5), O.dscale(1e10)))):
+        Y = np.zeros(2 * fold)
+        s = O.dbdssq(fold, x, sy, Y)
+        assert s == sy or sy < O.dscale(np.max(np.abs(x)))
+        for k in ("avx", "sisd"):
+            R = reference(k)
+            if R is not None:
+                Yr = np.zeros(2 * fold)
+                assert same(R.dbdssq(fold, x, sy, Yr), s)
+                assert np.array_equal(Yr.view(np.uint64), Y.view(np.uint64))
+        assert same(s, G[f"ssq0_{j}_f{fold}"][0])
+        assert np.array_equal(Y.view(np.uint64), G[f"ssq0_{j}_state_f{fold}"].view(np.uint64))
+
+
+def table_slots(fold, Y, O):
+    """Slots of a binned value whose bin index is inside the bin table
+    (index + slot <= 51, binned.h:105), for primaries and carries."""
+    I = O.dmindex(Y)
+    k = max(0, min(fold, 52 - I))
+    return np.r_[0:k, fold:fold + k]
+
+
+@pytest.mark.parametrize("fold", [28, 30, 40, 52])
+def test_ssq_slots_beyond_bin_table_depend_on_the_split(fold):
+    """At folds where a value's slots run past bin index 51 (binned.h:105;
+    the table repeats its last bin, dmbins.c:23-39), the scaled sum of
+    squares puts schedule-dependent content there: one dbdssq call and the
+    same terms in two calls merged by dbdbaddsq (dmdmaddsq.c:26-41) differ in
+    those slots only, and convert to the same result. The GPU results are
+    therefore compared on the slots inside the table at these folds."""
+    libs = [oracle()] + [r for r in (reference("avx"), reference("sisd")) if r is not None]
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    differs = 0
+    for L in libs:
+        for seed in range(6):
+            x = nrm2_range_input(seed)
+            Y = np.zeros(2 * fold)
+            L.dbdssq(fold, x, 0.0, Y)
+            A = np.zeros(2 * fold)
+            sa = L.dbdssq(fold, np.ascontiguousarray(x[:5000]), 0.0, A)
+            B = np.zeros(2 * fold)
+            sb = L.dbdssq(fold, np.ascontiguousarray(x[5000:]), 0.0, B)
+            L.fn["dbdbaddsq"](fold, sa, vp(A), sb, vp(B))
+            keep = table_slots(fold, Y, oracle())
+            assert np.array_equal(Y[keep].view(np.uint64), B[keep].view(np.uint64)), (fold, seed)
+            assert same(L.ddbconv(fold, Y), L.ddbconv(fold, B))
+            differs += not np.array_equal(Y.view(np.uint64), B.view(np.uint64))
+    assert differs > 0  # the slots past the table do depend on the split
