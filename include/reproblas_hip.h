@@ -62,6 +62,22 @@ int reproblas_hip_rdnrm2_device(int fold, int64_t N, const double *X, int64_t in
 int reproblas_hip_rddot_device(int fold, int64_t N, const double *X, int64_t incX, const double *Y, int64_t incY,
                                double *result, void *stream);
 
+/* reproblas_hip_rddot_device without allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, result and bits as reproblas_hip_rddot_device. Its scratch
+ * (workgroup partial records and the completion ticket) is a fixed 512 KiB
+ * device array of the library, one per GPU, kept until the process ends:
+ * enough for any fold with unit-stride operands and for folds up to 30 with
+ * strided ones; a call whose grid needs more is refused with
+ * REPROBLAS_HIP_BAD_ARGUMENT and nothing is launched. The call allocates
+ * nothing (the first call on a GPU looks up that GPU's array and clears its
+ * ticket with a one-thread kernel on `stream`). Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (different streams, or
+ * different host threads) race and must be serialised by the caller. */
+int reproblas_hip_rddot_device_noalloc(int fold, int64_t N, const double *X, int64_t incX, const double *Y,
+                                       int64_t incY, double *result, void *stream);
+
 /* Binned-state BLAS1 on device data: accumulate into the device binned
  * value Y (2*fold doubles), as binnedBLAS_dbdsum/dbdasum/dbddot do. For
  * dbdssq the scale lives in device memory too (*scale in, new scale out). */

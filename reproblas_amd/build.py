@@ -38,6 +38,7 @@ SOURCES = [
     "api.hip",
     "host_complex.hip",
     "api_complex.hip",
+    "noalloc.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
