@@ -30,6 +30,8 @@ SOURCES = [
       for t, name in enumerate(("sum", "asum", "dot", "ssq")) for ty in ("double", "float")],
     "complex_d.hip",
     "complex_f.hip",
+    *[("complex_inst.hip", {"RB_INST_TERM": str(t), "RB_INST_TYPE": ty}, f"complex_{name}_{ty[0]}")
+      for t, name in enumerate(("sum", "dotu", "dotc", "asum", "ssq")) for ty in ("double", "float")],
     "complex_mat.hip",
     "amax.hip",
     "gemv.hip",
@@ -94,7 +96,7 @@ def build(jobs=None, verbose=False, incremental=False):
     jobs = jobs or min(8, os.cpu_count() or 1)
     units = [u for u in SOURCES if os.path.exists(os.path.join(CSRC, _unit(u)[0]))]
     # longest units first, so the pool finishes together
-    heavy = ("blas1_inst", "complex_d", "complex_f", "gemm", "gemv")
+    heavy = ("blas1_", "complex_", "gemm", "gemv")
     units.sort(key=lambda u: not any(h in _unit(u)[2] for h in heavy))
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda f: _compile(f, incremental), units))

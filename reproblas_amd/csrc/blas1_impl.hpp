@@ -204,14 +204,26 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    State<CAP, F> acc, o;
-    load_state(lds[0], fold, acc);
-    for (int w = 1; w < kWaves; ++w) {
-      load_state(lds[w], fold, o);
-      merge<CAP, T>(fold, acc, o);
+  if constexpr (T == kSSq) {
+    // scaled values: the general scaled merge, one after another
+    if (threadIdx.x == 0) {
+      State<CAP, F> acc, o;
+      load_state(lds[0], fold, acc);
+      for (int w = 1; w < kWaves; ++w) {
+        load_state(lds[w], fold, o);
+        merge<CAP, T>(fold, acc, o);
+      }
+      store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, acc);
     }
-    store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, acc);
+  } else if (wave == 0) {
+    // the 4 wave values in lanes 0..3 of wave 0, aligned and added in 2 rounds
+    State<CAP, F> t;
+    const bool v = lane < kWaves;
+    if (v) load_state(lds[lane], fold, t);
+    wave_merge_aligned<2>(fold, v, t);
+    if (lane == 0) store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, t);
+  }
+  if (threadIdx.x == 0) {
     if (kInfPath) {
       int64_t fi = lds_fi[0];
       int big = lds_big[0];
@@ -222,8 +234,10 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
       a.aux[2 * (int64_t)blockIdx.x] = fi;
       a.aux[2 * (int64_t)blockIdx.x + 1] = big;
     }
-    __threadfence();
-    const unsigned ticket = atomicAdd(a.ticket, 1u);
+    // release the partial (and its record) to the last workgroup; the ticket
+    // itself is relaxed (fence-to-fence synchronisation through it)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned ticket = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (ticket == gridDim.x - 1);
   }
   __syncthreads();
@@ -231,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
 
   // Last workgroup: merge every partial (exact, so the order is free; it is
   // fixed anyway), fold in the caller's value, convert.
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   State<CAP, F> acc;
   bool have = false;
   int64_t fi = INT64_MAX;
@@ -247,9 +261,15 @@ RB_UNROLL_CAP
       }
     o.scl = __builtin_nontemporal_load(src + 2 * fold);
     if (kInfPath) {
-      const int64_t f = a.aux[2 * (int64_t)b];
-      fi = f < fi ? f : fi;
-      big |= a.aux[2 * (int64_t)b + 1] != 0;
+      // a finite partial is big when it holds index 0; only a poisoned one
+      // needs its record
+      if (nan_or_inf(o.p[0])) {
+        const int64_t f = a.aux[2 * (int64_t)b];
+        fi = f < fi ? f : fi;
+        big |= a.aux[2 * (int64_t)b + 1] != 0;
+      } else {
+        big |= dmindex0(o.p[0]);
+      }
     }
     if (!have) {
       acc = o;
@@ -258,13 +278,18 @@ RB_UNROLL_CAP
       merge<CAP, T>(fold, acc, o);
     }
   }
-  if (!have) {
-    set_bins(fold, Fmt<F>::max_index, acc.p, acc.c);
-    acc.scl = 0.0;
+  if constexpr (T == kSSq) {
+    if (!have) {
+      set_bins(fold, Fmt<F>::max_index, acc.p, acc.c);
+      acc.scl = 0.0;
+    }
+    wave_merge_general<CAP, T>(fold, acc);
+  } else {
+    wave_merge_aligned<6>(fold, have, acc);
   }
-  wave_merge_general<CAP, T>(fold, acc);
   if (kInfPath) {
-    fi = wave_min_i64(fi);
+    // only a poisoned value can need the replay: skip the reduction otherwise
+    if (nan_or_inf(uniform_value(acc.p[0]))) fi = wave_min_i64(fi);
     big = __any(big);
   }
   if (lane == 0) {
@@ -276,13 +301,43 @@ RB_UNROLL_CAP
   }
   __syncthreads();
   State<CAP, F> y;
-  __shared__ int64_t replay_at;  // first infinite term, or -1: no replay
+  if constexpr (T != kSSq) {
+    if (wave == 0) {
+      State<CAP, F> t;
+      const bool v = lane < kWaves;
+      if (v) load_state(lds[lane], fold, t);
+      wave_merge_aligned<2>(fold, v, t);
+      if (lane == 0) acc = t;
+    }
+  }
+  // Replay decision, made alike by every thread from the values in LDS (no
+  // further barrier): one infinity overall, a finite caller value, and a
+  // magnitude that can overflow a sequential sum.
+  int64_t pstar = -1;
+  if constexpr (kInfPath) {
+    int cls = 0;
+    int64_t f = INT64_MAX;
+    bool bg = false;
+    for (int w = 0; w < kWaves; ++w) {
+      const F p0 = lds[w][0];
+      if (nan_or_inf(p0)) cls |= class_bits(p0);
+      f = lds_fi[w] < f ? lds_fi[w] : f;
+      bg |= lds_big[w] != 0;
+    }
+    const bool one_inf = cls == 2 || cls == 4;
+    const F y0 = a.state_in ? a.state_in[0] : F(0);
+    // (f < n always holds for an infinite class; checked so no replay can
+    // ever index past the vector)
+    if (a.n > 0 && one_inf && f >= 0 && f < a.n && !nan_or_inf(y0) && (bg || dmindex0(y0))) pstar = f;
+  }
   if (threadIdx.x == 0) {
-    State<CAP, F> o;
-    load_state(lds[0], fold, acc);
-    for (int w = 1; w < kWaves; ++w) {
-      load_state(lds[w], fold, o);
-      merge<CAP, T>(fold, acc, o);
+    if constexpr (T == kSSq) {
+      State<CAP, F> o;
+      load_state(lds[0], fold, acc);
+      for (int w = 1; w < kWaves; ++w) {
+        load_state(lds[w], fold, o);
+        merge<CAP, T>(fold, acc, o);
+      }
     }
     // caller's value, renormalised first (the reference ends with a renorm)
     if (a.state_in) {
@@ -297,26 +352,9 @@ RB_UNROLL_CAP
       set_zero(fold, y.p, y.c);
     }
     y.scl = a.scale_in_dev ? *a.scale_in_dev : a.scale_in;
-    replay_at = -1;
-    if (kInfPath) {
-      int64_t f = lds_fi[0];
-      bool bg = lds_big[0] != 0;
-      for (int w = 1; w < kWaves; ++w) {
-        f = lds_fi[w] < f ? lds_fi[w] : f;
-        bg |= lds_big[w] != 0;
-      }
-      // one infinity, a finite caller value, and magnitudes that can overflow
-      const bool one_inf = nan_or_inf(acc.p[0]) && acc.p[0] == acc.p[0];
-      // (f < n always holds for an infinite class; checked so no replay can
-      // ever index past the vector)
-      if (a.n > 0 && one_inf && f >= 0 && f < a.n && !nan_or_inf(y.p[0]) && (bg || dmindex0(y.p[0])))
-        replay_at = f;
-    }
   }
   bool replayed = false;
   if constexpr (kInfPath) {
-    __syncthreads();
-    const int64_t pstar = replay_at;
     if (pstar >= 0) {
       __shared__ F seq[ref_block<T, F>()];
       // the reference's top primary after its first infinite block: it
@@ -328,10 +366,15 @@ RB_UNROLL_CAP
   }
   if (threadIdx.x == 0) {
     if (a.n > 0 && !replayed) {
-      // An empty caller value still carries its scale for nrm2: the reference
-      // squares every term at max(scaleY, dscale(amax)) (dmdssq.ccog:62-101),
-      // which the scaled merge reproduces by rescaling acc up to scaleY.
-      merge<CAP, T>(fold, y, acc);
+      if (a.state_in) {
+        // An empty caller value still carries its scale for nrm2: the
+        // reference squares every term at max(scaleY, dscale(amax))
+        // (dmdssq.ccog:62-101), which the scaled merge reproduces by
+        // rescaling acc up to scaleY.
+        merge<CAP, T>(fold, y, acc);
+      } else {
+        y = acc;  // what merging into the zero value (scale 0) gives
+      }
     }
     // n == 0: nothing deposited, the caller's value only (the reference's final renorm)
     renorm(fold, y.p, y.c);

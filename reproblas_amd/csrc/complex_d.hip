@@ -1,6 +1,6 @@
-// complex_d.hip -- the double complex instances of complex_impl.hpp
-// (zblas1, zgemv_rows for double); complex_f.hip holds the float ones, so
-// the two compile in parallel.
+// complex_d.hip -- the double complex entry of complex_impl.hpp (argument set-up
+// and grid of zblas1; the kernel instances are in the complex_inst.hip
+// units) and the double complex row-dot gemv.
 #include "complex_impl.hpp"
 
 namespace rb {

@@ -1,4 +1,6 @@
-// complex_f.hip -- the float complex instances of complex_impl.hpp.
+// complex_f.hip -- the float complex entry of complex_impl.hpp (argument set-up
+// and grid of zblas1; the kernel instances are in the complex_inst.hip
+// units) and the float complex row-dot gemv.
 #include "complex_impl.hpp"
 
 namespace rb {

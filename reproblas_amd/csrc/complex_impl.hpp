@@ -32,23 +32,6 @@
 
 namespace rb {
 
-namespace {
-
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-
-// Unit-stride workgroups per CU of the complex reductions and persistent
-// workgroups per CU of the complex row-dot gemv (compile-time overridable for
-// tuning builds). Measured on MI355X (2^27 double complex, fold 3): 1/CU zsum
-// 6.0, dznrm2 5.5, zdotc 6.2 TB/s; 2/CU 6.2, 6.1, 6.5 TB/s; the row-dot gemv
-// is within noise at 2, 4 and 8.
-#ifndef RB_ZBLAS1_BLOCKS_PER_CU
-#define RB_ZBLAS1_BLOCKS_PER_CU 2
-#endif
-#ifndef RB_ZROW_BLOCKS_PER_CU
-#define RB_ZROW_BLOCKS_PER_CU 4
-#endif
-
 template <class F>
 struct ZArgs {
   int64_t n;  // complex elements
@@ -66,6 +49,37 @@ struct ZArgs {
   F* result;  // 2 elements (sum/dot) or 1 (asum/nrm2)
   int fold;
 };
+
+// One launch of the complex BLAS1 kernel of a term on prepared arguments
+// (mode: 0 unit stride 16-byte aligned, 2 otherwise), compiled in units of
+// their own (complex_inst.hip built once per term and type).
+#define RB_ZLAUNCH_DECL(NAME)                                                    \
+  bool NAME(int fold, int mode, const ZArgs<double>& a, int blocks, hipStream_t s); \
+  bool NAME(int fold, int mode, const ZArgs<float>& a, int blocks, hipStream_t s);
+RB_ZLAUNCH_DECL(zblas1_launch_sum)
+RB_ZLAUNCH_DECL(zblas1_launch_dotu)
+RB_ZLAUNCH_DECL(zblas1_launch_dotc)
+RB_ZLAUNCH_DECL(zblas1_launch_asum)
+RB_ZLAUNCH_DECL(zblas1_launch_ssq)
+#undef RB_ZLAUNCH_DECL
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+// Unit-stride workgroups per CU of the complex reductions and persistent
+// workgroups per CU of the complex row-dot gemv (compile-time overridable for
+// tuning builds). Measured on MI355X (2^27 double complex, fold 3): 1/CU zsum
+// 6.0, dznrm2 5.5, zdotc 6.2 TB/s; 2/CU 6.2, 6.1, 6.5 TB/s; the row-dot gemv
+// is within noise at 2, 4 and 8.
+#ifndef RB_ZBLAS1_BLOCKS_PER_CU
+#define RB_ZBLAS1_BLOCKS_PER_CU 2
+#endif
+#ifndef RB_ZROW_BLOCKS_PER_CU
+#define RB_ZROW_BLOCKS_PER_CU 4
+#endif
+
 
 template <class F>
 struct ZSpan {
@@ -594,15 +608,12 @@ bool zlaunch_mode(int mode, const ZArgs<F>& a, int blocks, hipStream_t s) {
   return RB_HIP(hipGetLastError());
 }
 
-template <int CAP, bool FIXED, class F>
-bool zlaunch_term(ZTerm term, int mode, const ZArgs<F>& a, int blocks, hipStream_t s) {
-  switch (term) {
-    case kZSum: return zlaunch_mode<CAP, FIXED, kZSum>(mode, a, blocks, s);
-    case kZDotU: return zlaunch_mode<CAP, FIXED, kZDotU>(mode, a, blocks, s);
-    case kZDotC: return zlaunch_mode<CAP, FIXED, kZDotC>(mode, a, blocks, s);
-    case kZASum: return zlaunch_mode<CAP, FIXED, kZASum>(mode, a, blocks, s);
-    default: return zlaunch_mode<CAP, FIXED, kZSSq>(mode, a, blocks, s);
-  }
+// The kernel instance of term T for a fold (fold 3 has its own instance).
+template <int T, class F>
+bool zlaunch_fold_term(int fold, int mode, const ZArgs<F>& a, int blocks, hipStream_t s) {
+  if (fold == 3) return zlaunch_mode<3, true, T>(mode, a, blocks, s);
+  if (fold <= 8) return zlaunch_mode<8, false, T>(mode, a, blocks, s);
+  return zlaunch_mode<Fmt<F>::max_fold, false, T>(mode, a, blocks, s);
 }
 
 int64_t zunits_of(ZTerm term, size_t esize) {
@@ -654,9 +665,13 @@ bool zblas1_impl(int fold, ZTerm term, int64_t n, const F* x, int64_t incx, cons
   a.partials = (F*)workspace(s, kSlotPartials, sizeof(F) * (size_t)blocks * 2 * (2 * fold + 1));
   a.ticket = tickets(s, 1);
   if (!a.partials || !a.ticket) return false;
-  if (fold == 3) return zlaunch_term<3, true>(term, mode, a, (int)blocks, s);
-  if (fold <= 8) return zlaunch_term<8, false>(term, mode, a, (int)blocks, s);
-  return zlaunch_term<Fmt<F>::max_fold, false>(term, mode, a, (int)blocks, s);
+  switch (term) {
+    case kZSum: return zblas1_launch_sum(fold, mode, a, (int)blocks, s);
+    case kZDotU: return zblas1_launch_dotu(fold, mode, a, (int)blocks, s);
+    case kZDotC: return zblas1_launch_dotc(fold, mode, a, (int)blocks, s);
+    case kZASum: return zblas1_launch_asum(fold, mode, a, (int)blocks, s);
+    default: return zblas1_launch_ssq(fold, mode, a, (int)blocks, s);
+  }
 }
 
 }  // namespace

@@ -39,10 +39,18 @@ std::recursive_mutex* stream_mutex(hipStream_t s) {
     (void)hipGetLastError();
     dev = -1;
   }
+  // the last (device, stream) this thread used: no global lock on repeated calls
+  thread_local int last_dev = -2;
+  thread_local hipStream_t last_s = nullptr;
+  thread_local std::recursive_mutex* last_m = nullptr;
+  if (last_m && last_dev == dev && last_s == s) return last_m;
   std::lock_guard<std::mutex> lk(g_mu);
   static auto* m = new std::map<std::pair<int, hipStream_t>, std::recursive_mutex*>();
   std::recursive_mutex*& p = (*m)[std::make_pair(dev, s)];
   if (!p) p = new std::recursive_mutex();
+  last_dev = dev;
+  last_s = s;
+  last_m = p;
   return p;
 }
 }  // namespace

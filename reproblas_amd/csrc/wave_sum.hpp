@@ -24,6 +24,7 @@
 //
 // Included into an anonymous namespace of each kernel source.
 #pragma once
+#include <limits.h>
 
 #include "binned_core.hpp"
 #include "kernels.hpp"
@@ -237,6 +238,66 @@ RB_UNROLL_CAP
   }
 }
 
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+  for (int st = 0; st < 6; ++st) {
+    const int o = lane_partner32(v, st);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_or_int(int v) {
+#pragma unroll
+  for (int st = 0; st < 6; ++st) v |= lane_partner32(v, st);
+  return v;
+}
+
+// Class of a non-finite value as bits (1 NaN, 2 +inf, 4 -inf), and back: the
+// class that adding the values would give (NaN for any NaN or both
+// infinities, else the one infinity).
+template <class F>
+__device__ __forceinline__ int class_bits(F v) {
+  return (v != v) ? 1 : (v > F(0) ? 2 : 4);
+}
+template <class F>
+__device__ __forceinline__ F class_value(int b) {
+  if ((b & 1) || (b & 6) == 6) return (F)__builtin_nan("");
+  return (b & 2) ? (F)__builtin_inf() : -(F)__builtin_inf();
+}
+
+// Exact merge of renormalised values held by the lanes of a wave (lanes with
+// `valid` false hold nothing); afterwards every lane of each aligned group of
+// 2^STAGES lanes holds its group's sum (STAGES = 6: the whole wave's). The
+// values may differ in index: each is first raised to the wave's lowest
+// index -- the alignment the general add (dmdmadd.c:27-74) makes pairwise,
+// exact because a raise only shifts slots -- and then the groups combine by
+// same-index adds. Non-finite values combine by class, as adding their top
+// primaries would. Not for scaled (nrm2) values.
+template <int STAGES, int CAP, class F>
+__device__ __forceinline__ void wave_merge_aligned(int fold, bool valid, State<CAP, F>& s) {
+  const bool fin = valid && !nan_or_inf(s.p[0]);
+  const int mine = fin ? dmindex(s.p[0]) : INT_MAX;
+  int I = uniform_int(wave_min_int(mine));
+  const int cls = uniform_int(wave_or_int((valid && !fin) ? class_bits(s.p[0]) : 0));
+  if (I == INT_MAX) I = Fmt<F>::max_index;
+  if (fin) {
+    raise_index(fold, mine - I, I, s.p, s.c);
+  } else {
+    set_bins(fold, I, s.p, s.c);
+  }
+#pragma unroll
+  for (int st = 0; st < STAGES; ++st) {
+    F op[CAP], oc[CAP];
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j) {
+      op[j] = lane_partner(s.p[j], st);
+      oc[j] = lane_partner(s.c[j], st);
+    }
+    add_same_index(fold, I, s.p, s.c, op, oc);
+  }
+  if (cls) s.p[0] = class_value<F>(cls);
+}
+
 // Merge the 64 lanes of a wave (general add: lanes may differ in index).
 template <int CAP, int K, class F>
 __device__ __forceinline__ void wave_merge_general(int fold, State<CAP, F>& s) {
@@ -264,10 +325,15 @@ struct PoisonInfo {
   bool big;
 };
 
+// Minimum over the wave (every lane ends with it), by the lane moves of the
+// reduction trees below on the two halves.
 __device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
 #pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-    const int64_t o = __shfl_xor((long long)v, m, 64);
+  for (int st = 0; st < 6; ++st) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)lane_partner32((int)(uint32_t)u, st);
+    const uint32_t hi = (uint32_t)lane_partner32((int)(uint32_t)(u >> 32), st);
+    const int64_t o = (int64_t)(((uint64_t)hi << 32) | lo);
     v = o < v ? o : v;
   }
   return v;
