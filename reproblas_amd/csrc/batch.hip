@@ -112,6 +112,31 @@ bool launch_all(reproblas_hip_batch* b, hipStream_t s) {
   return ok;
 }
 
+void free_group(SegGroup& g) {
+  (void)hipFree(g.d_chunks);
+  (void)hipFree(g.d_first);
+  (void)hipFree(g.d_partials);
+  g.d_chunks = nullptr;
+  g.d_first = nullptr;
+  g.d_partials = nullptr;
+}
+
+void free_group(GemmGroup& g) {
+  (void)hipFree(g.d_descs);
+  (void)hipFree(g.d_tiles);
+  (void)hipFree(g.d_maxima);
+  g.d_descs = nullptr;
+  g.d_tiles = nullptr;
+  g.d_maxima = nullptr;
+}
+
+// Frees a group that failed to build and returns the error already set.
+template <class G>
+int fail_group(G& g) {
+  free_group(g);
+  return last_error();
+}
+
 void drop_graph(reproblas_hip_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -185,11 +210,11 @@ int reproblas_hip_batch_add_segments(reproblas_hip_batch* b, int op, int64_t nse
   g.nchunks = (int64_t)chunks.size();
   g.d_chunks = upload(chunks);
   g.d_first = upload(first);
-  if ((g.nchunks && !g.d_chunks) || !g.d_first) return last_error();
+  if ((g.nchunks && !g.d_chunks) || !g.d_first) return fail_group(g);
   if (g.nchunks && !RB_HIP(hipMalloc((void**)&g.d_partials, sizeof(double) * g.nchunks * (2 * b->fold + 1))))
-    return last_error();
+    return fail_group(g);
   const DeviceInfo* d = device_info();
-  if (!d) return last_error();
+  if (!d) return fail_group(g);
   int64_t grid = (g.nchunks + 3) / 4;
   grid = std::min<int64_t>(grid, (int64_t)d->cus * kSegBlocksPerCU);
   g.grid = (int)std::max<int64_t>(grid, 1);
@@ -254,9 +279,9 @@ int reproblas_hip_batch_add_gemms(reproblas_hip_batch* b, char Order, char Trans
   g.desc0 = descs[0];
   g.d_descs = upload(descs);
   g.d_tiles = upload(tiles);
-  if (!g.d_descs || !g.d_tiles) return last_error();
+  if (!g.d_descs || !g.d_tiles) return fail_group(g);
   const DeviceInfo* d = device_info();
-  if (!d) return last_error();
+  if (!d) return fail_group(g);
   g.grid = (int)std::min<int64_t>(g.ntiles, INT_MAX);  // one workgroup per tile
   b->gemms.push_back(g);
   return kOk;
@@ -308,16 +333,8 @@ void reproblas_hip_batch_destroy(reproblas_hip_batch* b) {
   if (b->side) (void)hipStreamDestroy(b->side);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
-  for (SegGroup& g : b->segs) {
-    (void)hipFree(g.d_chunks);
-    (void)hipFree(g.d_first);
-    (void)hipFree(g.d_partials);
-  }
-  for (GemmGroup& g : b->gemms) {
-    (void)hipFree(g.d_descs);
-    (void)hipFree(g.d_tiles);
-    (void)hipFree(g.d_maxima);
-  }
+  for (SegGroup& g : b->segs) free_group(g);
+  for (GemmGroup& g : b->gemms) free_group(g);
   if (prev >= 0) (void)hipSetDevice(prev);
   delete b;
 }

@@ -314,7 +314,11 @@ template <int CAP, bool FIXED, int TM, int TN, bool AI, bool BK, class F>
 __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDescT<F> g) {
   __shared__ __attribute__((aligned(16))) F As[KC][16 * TM + pad<F>()];
   __shared__ __attribute__((aligned(16))) F Bs[KC][16 * TN + pad<F>()];
-  gemm_tile<CAP, FIXED, TM, TN, AI, BK>(g, (int64_t)blockIdx.y * 16 * TM, (int64_t)blockIdx.x * 16 * TN, As, Bs);
+  // one linear tile index (row tiles on grid.y would cap M at 65535 tiles);
+  // column tiles vary fastest, the dispatch order of a 2-D grid
+  const unsigned tiles_n = (unsigned)((g.N + 16 * TN - 1) / (16 * TN));
+  const unsigned ti = blockIdx.x / tiles_n, tj = blockIdx.x - ti * tiles_n;
+  gemm_tile<CAP, FIXED, TM, TN, AI, BK>(g, (int64_t)ti * 16 * TM, (int64_t)tj * 16 * TN, As, Bs);
 }
 
 // Ragged gemm batch: one workgroup per entry of a tile list (gemm, tile row,
@@ -353,7 +357,12 @@ __global__ __launch_bounds__(256) void chunk_max_batch_kernel(const GemmDesc* de
 
 template <int CAP, bool FIXED, int TM, int TN, class F>
 bool launch_tm(bool ai, bool bk, const GemmDescT<F>& g, hipStream_t s) {
-  dim3 grid((unsigned)((g.N + 16 * TN - 1) / (16 * TN)), (unsigned)((g.M + 16 * TM - 1) / (16 * TM)));
+  const int64_t tiles = ((g.N + 16 * TN - 1) / (16 * TN)) * ((g.M + 16 * TM - 1) / (16 * TM));
+  if (tiles > INT_MAX) {
+    set_error(kBadArgument, "gemm of %lld output tiles exceeds one launch", (long long)tiles);
+    return false;
+  }
+  const dim3 grid((unsigned)tiles);
   if (ai && bk) gemm_kernel<CAP, FIXED, TM, TN, true, true, F><<<grid, kThreads, 0, s>>>(g);
   else if (ai) gemm_kernel<CAP, FIXED, TM, TN, true, false, F><<<grid, kThreads, 0, s>>>(g);
   else if (bk) gemm_kernel<CAP, FIXED, TM, TN, false, true, F><<<grid, kThreads, 0, s>>>(g);

@@ -492,6 +492,38 @@ def test_full_size_dgemm_per_output(gpu):
         rb.set_gemm_tiling(old)
 
 
+# ------------------------------------------------ tall-skinny dgemm --
+@pytest.mark.slow
+def test_tall_dgemm_beyond_65535_tile_rows(gpu):
+    """rdgemm over 2^22 + 37 rows (more than 65535 row tiles at every tiling,
+    past the y-dimension limit of a launch grid), N = 3, K = 5, column-major:
+    the whole result equals the oracle's, at each tiling."""
+    import torch
+
+    import reproblas_amd as rb
+
+    M, N, K = (1 << 22) + 37, 3, 5
+    A = dg.normal(M * K, 611)
+    B = dg.normal(K * N, 612)
+    C0 = dg.normal(M * N, 613)
+    Ce = C0.copy()
+    oracle().fn["rdgemm"](3, b"C", b"N", b"N", M, N, K, 1.0, vp(A), M, vp(B), K, 0.5, vp(Ce), M)
+    tA, tB = torch.from_numpy(A).to(gpu), torch.from_numpy(B).to(gpu)
+    L = rb.lib()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    old = rb.get_gemm_tiling()
+    try:
+        for tiling in (0, 1, 2):
+            rb.set_gemm_tiling(tiling)
+            tC = torch.from_numpy(C0).to(gpu)
+            assert L.reproblas_hip_rdgemm_device(3, b"C", b"N", b"N", M, N, K, 1.0, dptr(tA), M, dptr(tB), K, 0.5,
+                                                 dptr(tC), M, s) == 0, L.reproblas_hip_last_error_string()
+            torch.cuda.synchronize()
+            assert np.array_equal(u64(tC.cpu().numpy()), u64(Ce)), tiling
+    finally:
+        rb.set_gemm_tiling(old)
+
+
 # ------------------------------------ full-size ragged batch, sampled --
 @pytest.mark.slow
 def test_full_size_ragged_batch(gpu):
