@@ -272,6 +272,42 @@ RB_UNROLL_CAP
     if (j == fold - 1) p[j] = p[j] + set_lsb(x);
 }
 
+// Two terms into two independent sets of primaries (each exactly as
+// deposit() above), written step by step in lockstep so that the two
+// dependency chains interleave in the instruction stream.
+template <int CAP, class T>
+RB_HD void deposit2(int fold, bool top, T xa, T (&pa)[CAP], T xb, T (&pb)[CAP]) {
+  if (top) {
+    const T ma = pa[0], mb = pb[0];
+    const T qa = ma + set_lsb(xa * Fmt<T>::compression);
+    const T qb = mb + set_lsb(xb * Fmt<T>::compression);
+    pa[0] = qa;
+    pb[0] = qb;
+    const T ra = (ma - qa) * (Fmt<T>::expansion * T(0.5));
+    const T rb = (mb - qb) * (Fmt<T>::expansion * T(0.5));
+    xa = (xa + ra) + ra;
+    xb = (xb + rb) + rb;
+  }
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j) {
+    if (j < fold - 1 && !(top && j == 0)) {
+      const T ma = pa[j], mb = pb[j];
+      const T qa = ma + set_lsb(xa);
+      const T qb = mb + set_lsb(xb);
+      pa[j] = qa;
+      pb[j] = qb;
+      xa = xa + (ma - qa);
+      xb = xb + (mb - qb);
+    }
+  }
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j == fold - 1) {
+      pa[j] = pa[j] + set_lsb(xa);
+      pb[j] = pb[j] + set_lsb(xb);
+    }
+}
+
 // reference src/binned/dmrenorm.c:70-85, smrenorm.c: move the two bits below
 // the leading one into the carry so p returns to [1.5, 1.75) * ufp.
 template <int CAP, class T>

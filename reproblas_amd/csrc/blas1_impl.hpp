@@ -66,6 +66,49 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 
+// The sum of the kWaves renormalised values vals[0..kWaves) (records of
+// 2*fold+1 in LDS), slot-parallel: the values join their lowest index
+// (integer form, wave_sum.hpp) and lane j of the calling wave sums the
+// offsets and carries of slot j over the values. Every lane of the wave
+// calls it; lane 0 ends with the renormalised sum in `out` (top primary the
+// class of the non-finite values, if any). Not for scaled (nrm2) values.
+template <int CAP, class F>
+__device__ __forceinline__ void merge_slots(int fold, const F (*vals)[2 * CAP + 1], int lane, State<CAP, F>& out) {
+  using Ubits = typename Fmt<F>::U;
+  int I = INT_MAX, cls = 0;
+  int idx[kWaves];
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const F p0 = vals[w][0];
+    idx[w] = nan_or_inf(p0) ? INT_MAX : dmindex(p0);
+    if (nan_or_inf(p0)) cls |= class_bits(p0);
+    I = idx[w] < I ? idx[w] : I;
+  }
+  if (I == INT_MAX) I = Fmt<F>::max_index;
+  const int j = lane < fold ? lane : 0;
+  const int64_t b = (int64_t)bits(bin<F>(I + j));
+  int64_t d = 0;
+  F c = F(0);
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const int src = j - (idx[w] - I);  // the value's own slot that lands on slot j
+    if (idx[w] != INT_MAX && src >= 0) {
+      d += (int64_t)bits(vals[w][src]) - b;
+      c += vals[w][fold + src];
+    }
+  }
+  constexpr int lo = Lay<F>::mbits - 2;
+  const F pj = fp_from_bits<F>((Ubits)(b + (d & (((int64_t)1 << lo) - 1))));
+  const F cj = c + (F)(d >> lo);
+RB_UNROLL_CAP
+  for (int k = 0; k < CAP; ++k)
+    if (k < fold) {
+      out.p[k] = lane_value(pj, k);
+      out.c[k] = lane_value(cj, k);
+    }
+  if (cls) out.p[0] = class_value<F>(cls);
+}
+
 
 // Block length of the reference's driver for a term kind: binnedBLAS_dmdsum
 // and dmddot walk blocks of 2048 and 1024 doubles (dmdsum.ccog:36,
@@ -216,11 +259,10 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
       store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, acc);
     }
   } else if (wave == 0) {
-    // the 4 wave values in lanes 0..3 of wave 0, aligned and added in 2 rounds
+    // the 4 wave values, slot-parallel in wave 0
     State<CAP, F> t;
-    const bool v = lane < kWaves;
-    if (v) load_state(lds[lane], fold, t);
-    wave_merge_aligned<2>(fold, v, t);
+    merge_slots(fold, lds, lane, t);
+    t.scl = F(0);
     if (lane == 0) store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, t);
   }
   if (threadIdx.x == 0) {
@@ -304,9 +346,8 @@ RB_UNROLL_CAP
   if constexpr (T != kSSq) {
     if (wave == 0) {
       State<CAP, F> t;
-      const bool v = lane < kWaves;
-      if (v) load_state(lds[lane], fold, t);
-      wave_merge_aligned<2>(fold, v, t);
+      merge_slots(fold, lds, lane, t);
+      t.scl = F(0);
       if (lane == 0) acc = t;
     }
   }

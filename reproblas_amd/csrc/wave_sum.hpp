@@ -81,6 +81,17 @@ __device__ __forceinline__ float uniform_value(float v) {
 }
 __device__ __forceinline__ double uniform_double(double v) { return uniform_value(v); }
 
+// The value of lane k (k wave-uniform).
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const uint64_t u = bits(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, k);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), k);
+  return from_bits(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float lane_value(float v, int k) {
+  return fp_from_bits<float>((uint32_t)__builtin_amdgcn_readlane((int)bits(v), k));
+}
+
 template <class F>
 __device__ __forceinline__ F wave_max(F v) {
 #pragma unroll
@@ -252,6 +263,64 @@ __device__ __forceinline__ int wave_or_int(int v) {
   return v;
 }
 
+// Minimum and OR over each aligned group of 2^STAGES lanes.
+template <int STAGES>
+__device__ __forceinline__ int group_min_int(int v) {
+#pragma unroll
+  for (int st = 0; st < STAGES; ++st) {
+    const int o = lane_partner32(v, st);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+template <int STAGES>
+__device__ __forceinline__ int group_or_int(int v) {
+#pragma unroll
+  for (int st = 0; st < STAGES; ++st) v |= lane_partner32(v, st);
+  return v;
+}
+
+__device__ __forceinline__ int64_t lane_partner_i64(int64_t v, int st) {
+  const uint64_t u = (uint64_t)v;
+  const uint32_t lo = (uint32_t)lane_partner32((int)(uint32_t)u, st);
+  const uint32_t hi = (uint32_t)lane_partner32((int)(uint32_t)(u >> 32), st);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// ---- exact merges in integer form ----
+// A value's slots at index I as integers: the offset of each primary from
+// its bin, in units of that bin's ulp (exact: between renormalisations a
+// primary stays in its bin's binade, so the offset is a difference of bit
+// patterns), next to the carries. Renormalisation keeps the low
+// `mbits - 2` bits of the offset in the primary and moves the rest, as a
+// floor division, into the carry (src/binned/dmrenorm.c:70-85). Values at
+// one index therefore merge exactly by summing offsets and carries, and the
+// renormalised sum is a function of those sums alone: equal to any order of
+// exact adds (src/binned/dmdmadd.c:27-74) followed by a renorm. A value at a
+// higher index joins a lower index I by moving its slots down (slot j of the
+// raised value is its slot j - shift, fresh top slots are offset 0), which
+// is what the index raise of dmdupdate.c:42-55 does.
+template <int CAP, class F>
+__device__ __forceinline__ void to_offsets(int fold, int I, const F (&p)[CAP], int64_t (&d)[CAP]) {
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) d[j] = (int64_t)bits(p[j]) - (int64_t)bits(bin<F>(I + j));
+}
+
+// The renormalised value at index I whose slot offsets sum to d; the
+// carries c already hold the carry sums and receive the overflow of d.
+template <int CAP, class F>
+__device__ __forceinline__ void from_offsets(int fold, int I, const int64_t (&d)[CAP], F (&p)[CAP], F (&c)[CAP]) {
+  using U = typename Fmt<F>::U;
+  constexpr int lo = Lay<F>::mbits - 2;
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) {
+      p[j] = fp_from_bits<F>((U)(bits(bin<F>(I + j)) + (U)(d[j] & (((int64_t)1 << lo) - 1))));
+      c[j] += (F)(d[j] >> lo);
+    }
+}
+
 // Class of a non-finite value as bits (1 NaN, 2 +inf, 4 -inf), and back: the
 // class that adding the values would give (NaN for any NaN or both
 // infinities, else the one infinity).
@@ -268,33 +337,41 @@ __device__ __forceinline__ F class_value(int b) {
 // Exact merge of renormalised values held by the lanes of a wave (lanes with
 // `valid` false hold nothing); afterwards every lane of each aligned group of
 // 2^STAGES lanes holds its group's sum (STAGES = 6: the whole wave's). The
-// values may differ in index: each is first raised to the wave's lowest
-// index -- the alignment the general add (dmdmadd.c:27-74) makes pairwise,
-// exact because a raise only shifts slots -- and then the groups combine by
-// same-index adds. Non-finite values combine by class, as adding their top
-// primaries would. Not for scaled (nrm2) values.
+// values may differ in index: each joins the group's lowest index -- the
+// alignment the general add (dmdmadd.c:27-74) makes pairwise -- and the
+// group sums its slot offsets and carries (integer form above). Non-finite
+// values combine by class, as adding their top primaries would. Not for
+// scaled (nrm2) values.
 template <int STAGES, int CAP, class F>
 __device__ __forceinline__ void wave_merge_aligned(int fold, bool valid, State<CAP, F>& s) {
   const bool fin = valid && !nan_or_inf(s.p[0]);
-  const int mine = fin ? dmindex(s.p[0]) : INT_MAX;
-  int I = uniform_int(wave_min_int(mine));
-  const int cls = uniform_int(wave_or_int((valid && !fin) ? class_bits(s.p[0]) : 0));
+  const int mine = fin ? dmindex(s.p[0]) : Fmt<F>::max_index;
+  int I = group_min_int<STAGES>(fin ? mine : INT_MAX);
+  const int cls = group_or_int<STAGES>((valid && !fin) ? class_bits(s.p[0]) : 0);
   if (I == INT_MAX) I = Fmt<F>::max_index;
-  if (fin) {
-    raise_index(fold, mine - I, I, s.p, s.c);
-  } else {
-    set_bins(fold, I, s.p, s.c);
-  }
+  int64_t own[CAP], d[CAP];
+  F c[CAP];
+  to_offsets(fold, mine, s.p, own);
+  const int sh = fin ? mine - I : CAP;  // slot j takes own slot j - sh
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) {
+      d[j] = (j >= sh) ? pick(own, j - sh) : 0;
+      c[j] = (j >= sh) ? pick(s.c, j - sh) : F(0);
+    }
 #pragma unroll
   for (int st = 0; st < STAGES; ++st) {
-    F op[CAP], oc[CAP];
 RB_UNROLL_CAP
-    for (int j = 0; j < CAP; ++j) {
-      op[j] = lane_partner(s.p[j], st);
-      oc[j] = lane_partner(s.c[j], st);
-    }
-    add_same_index(fold, I, s.p, s.c, op, oc);
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        d[j] += lane_partner_i64(d[j], st);
+        c[j] += lane_partner(c[j], st);
+      }
   }
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) s.c[j] = c[j];
+  from_offsets(fold, I, d, s.p, s.c);
   if (cls) s.p[0] = class_value<F>(cls);
 }
 
@@ -383,20 +460,31 @@ struct WaveRun {
   int dep;  // deposits since the last renormalisation
 };
 
-// Deposits one tile of raw terms (cx, cy) into the lane's bins.
+// Deposits one tile of raw terms (cx, cy) into the lane's bins. The even
+// terms go into the lane's primaries and the odd ones into a second set that
+// starts the tile at the same bins, so the two dependency chains interleave;
+// the second set's offsets then join the first (exact: together they hold
+// exactly the slices one set would, within the endurance).
 template <int CAP, int K, int U, class F>
 __device__ __forceinline__ void deposit_tile(int fold, const F (&cx)[U], const F (&cy)[U], WaveRun<F>& w,
                                              State<CAP, F>& s) {
   F v[U];
-  F m = F(0);
 #pragma unroll
   for (int k = 0; k < U; ++k) {
     if (K == kSum) v[k] = cx[k];
     if (K == kASum) v[k] = fabs(cx[k]);
     if (K == kDot) v[k] = cx[k] * cy[k];
     if (K == kSSq) v[k] = cx[k];
-    m = fmax(m, fabs(v[k]));
   }
+  // max |term| by a balanced tree (NaN ignored, as fmax does)
+  F t[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) t[k] = fabs(v[k]);
+#pragma unroll
+  for (int n = U; n > 1; n = (n + 1) / 2)
+#pragma unroll
+    for (int k = 0; k + (n + 1) / 2 < n; ++k) t[k] = fmax(t[k], t[k + (n + 1) / 2]);
+  const F m = fmax(F(0), t[0]);
   bool need;
   if (K == kSSq) {
     const F ms = m * w.inv;
@@ -431,21 +519,30 @@ __device__ __forceinline__ void deposit_tile(int fold, const F (&cx)[U], const F
     }
     w.lim = admit_limit<F>(w.idx);
   }
-  if (w.idx == 0) {
+  F q[CAP];
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) q[j] = bin<F>(w.idx + j);
+  if (K == kSSq) {
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      F tv = v[k];
-      if (K == kSSq) { tv = v[k] * w.inv; tv = tv * tv; }
-      deposit(fold, true, tv, s.p);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      F tv = v[k];
-      if (K == kSSq) { tv = v[k] * w.inv; tv = tv * tv; }
-      deposit(fold, false, tv, s.p);
+      v[k] = v[k] * w.inv;
+      v[k] = v[k] * v[k];
     }
   }
+  const bool top = (w.idx == 0);
+  if (top) {
+#pragma unroll
+    for (int k = 0; k + 1 < U; k += 2) deposit2(fold, true, v[k], s.p, v[k + 1], q);
+    if (U % 2) deposit(fold, true, v[U - 1], s.p);
+  } else {
+#pragma unroll
+    for (int k = 0; k + 1 < U; k += 2) deposit2(fold, false, v[k], s.p, v[k + 1], q);
+    if (U % 2) deposit(fold, false, v[U - 1], s.p);
+  }
+RB_UNROLL_CAP
+  for (int j = 0; j < CAP; ++j)
+    if (j < fold) s.p[j] = s.p[j] + (q[j] - bin<F>(w.idx + j));
   w.dep += U;
 }
 
@@ -464,29 +561,29 @@ __device__ __forceinline__ void wave_start(int fold, WaveRun<F>& w, State<CAP, F
   w.dep = 0;
 }
 
-// Ends a wave's reduction of the tiles t0, t0 + tstep, ... of `a`: renorm,
-// exact same-index butterfly over the lanes (they share the index and scale),
-// and, if the value is poisoned, the class of the terms.
-// Renorm and exact same-index butterfly over the 64 lanes (they share the
-// index and scale); a poisoned value stays poisoned.
+// Exact same-index merge of the 64 lanes (they share the wave's index and
+// scale) in integer form: every lane ends with the wave's renormalised value.
+// A poisoned lane poisons the wave (the caller then replaces the top primary
+// by the class of the terms).
 template <int CAP, class F>
 __device__ __forceinline__ void wave_butterfly(int fold, const WaveRun<F>& w, State<CAP, F>& s) {
-  renorm(fold, s.p, s.c);
   s.scl = w.scl;
+  if (__any(nan_or_inf(s.p[0]))) {
+    s.p[0] = (F)__builtin_nan("");
+    return;
+  }
+  int64_t d[CAP];
+  to_offsets(fold, w.idx, s.p, d);
 #pragma unroll
   for (int st = 0; st < 6; ++st) {
-    F op[CAP], oc[CAP];
 RB_UNROLL_CAP
-    for (int j = 0; j < CAP; ++j) {
-      op[j] = lane_partner(s.p[j], st);
-      oc[j] = lane_partner(s.c[j], st);
-    }
-    if (!nan_or_inf(s.p[0]) && !nan_or_inf(op[0])) {
-      add_same_index(fold, w.idx, s.p, s.c, op, oc);
-    } else {
-      s.p[0] = s.p[0] + op[0];  // keep the poison visible
-    }
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        d[j] += lane_partner_i64(d[j], st);
+        s.c[j] += lane_partner(s.c[j], st);
+      }
   }
+  from_offsets(fold, w.idx, d, s.p, s.c);
 }
 
 template <int CAP, int K, int U, int MODE, class F>
