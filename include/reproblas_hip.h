@@ -66,15 +66,14 @@ int reproblas_hip_rddot_device(int fold, int64_t N, const double *X, int64_t inc
  * reference; for callers that cannot allocate device memory (code under
  * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Same arguments, result and bits as reproblas_hip_rddot_device. Its scratch
- * (workgroup partial records and the completion ticket) is a fixed 512 KiB
- * device array of the library, one per GPU, kept until the process ends:
- * enough for any fold with unit-stride operands and for folds up to 30 with
- * strided ones; a call whose grid needs more is refused with
- * REPROBLAS_HIP_BAD_ARGUMENT and nothing is launched. The call allocates
- * nothing (the first call on a GPU looks up that GPU's array and clears its
- * ticket with a one-thread kernel on `stream`). Every call on a GPU shares
- * that GPU's array: calls that overlap on one GPU (different streams, or
- * different host threads) race and must be serialised by the caller. */
+ * (the completion ticket and the accumulators through which the workgroups
+ * merge) is a fixed 64 KiB device array of the library, one per GPU, kept
+ * until the process ends; it does not depend on N, the strides or the fold,
+ * so no size is refused. The call allocates nothing (the first call on a GPU
+ * looks up that GPU's array and clears its sync block with a one-workgroup
+ * kernel on `stream`). Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (different streams, or different host threads)
+ * race and must be serialised by the caller. */
 int reproblas_hip_rddot_device_noalloc(int fold, int64_t N, const double *X, int64_t incX, const double *Y,
                                        int64_t incY, double *result, void *stream);
 

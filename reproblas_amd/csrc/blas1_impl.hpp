@@ -29,6 +29,31 @@
 
 namespace rb {
 
+// Cross-workgroup accumulators of the sum, asum and dot kernels, in the
+// zeroed block that also holds the completion ticket (at byte kGridAccOffset;
+// every launch leaves it zero again). Each workgroup adds its value's slot
+// offsets and carries (integer form, wave_sum.hpp) at their absolute bin
+// indexes -- index + slot, the same bin for every value -- so the last
+// workgroup reads the sums aligned to the lowest index directly: a higher
+// value's slots below that index's window are the ones its raise would
+// drop. Integer atomics are associative, so the arrival order is free.
+// Workgroup b adds into copy b % kCopies: atomics on one address serialise
+// (~20 ns each), so the copies keep a 128-workgroup grid at 8 per address.
+struct GridAcc {
+  static constexpr int kCopies = 16;
+  static constexpr int kBins = 104;  // >= max_index + max_fold (double: 103; float: 41)
+  int64_t d[kCopies][kBins];         // slot offsets by absolute bin
+  int64_t c[kCopies][kBins];         // carries by absolute bin
+  int top[kCopies];                  // max of max_index - index (finite values)
+  int flags;                         // class bits of poisoned values | 8: one saw a finite term of index 0
+  int pad;
+  int64_t fi;                        // max of INT64_MAX - first infinite term (poisoned values)
+};
+constexpr size_t kGridAccOffset = 256;
+constexpr size_t kSyncBytes = kGridAccOffset + sizeof(GridAcc);  // ticket block of one launch
+constexpr int kGridAccWords = (int)(sizeof(GridAcc) / sizeof(int64_t));
+static_assert(kSyncBytes <= kBlas1SyncBytes, "the caller-provided sync block must hold the grid accumulators");
+
 template <class F>
 struct Blas1Args {
   int64_t n;
@@ -36,9 +61,8 @@ struct Blas1Args {
   int64_t incx;
   const F* y;
   int64_t incy;
-  F* partials;    // gridDim.x records of (2*fold+1) elements
-  int64_t* aux;   // sum / dot: gridDim.x records {first infinite term, big} (PoisonInfo)
-  unsigned* ticket;
+  F* partials;    // nrm2: gridDim.x records of (2*fold+1) elements
+  unsigned* ticket;  // zeroed block of kSyncBytes: the ticket, then a GridAcc
   const F* state_in;
   F scale_in;
   const F* scale_in_dev;  // when set, the scale of state_in is read here
@@ -66,16 +90,17 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 
-// The sum of the kWaves renormalised values vals[0..kWaves) (records of
-// 2*fold+1 in LDS), slot-parallel: the values join their lowest index
-// (integer form, wave_sum.hpp) and lane j of the calling wave sums the
-// offsets and carries of slot j over the values. Every lane of the wave
-// calls it; lane 0 ends with the renormalised sum in `out` (top primary the
-// class of the non-finite values, if any). Not for scaled (nrm2) values.
+// Slot sums of the kWaves renormalised values vals[0..kWaves) (records of
+// 2*fold+1 in LDS): the finite values join their lowest index I (integer
+// form, wave_sum.hpp) and lane j of the calling wave sums the offsets (d) and
+// carries (c) of slot j over them; cls gathers the class bits of the
+// non-finite values. Every lane of the wave calls it; I and cls are uniform.
+// Not for scaled (nrm2) values.
 template <int CAP, class F>
-__device__ __forceinline__ void merge_slots(int fold, const F (*vals)[2 * CAP + 1], int lane, State<CAP, F>& out) {
-  using Ubits = typename Fmt<F>::U;
-  int I = INT_MAX, cls = 0;
+__device__ __forceinline__ void slot_sums(int fold, F (*vals)[2 * CAP + 1], int lane, int& I, int& cls,
+                                          int64_t& d, F& c) {
+  I = INT_MAX;
+  cls = 0;
   int idx[kWaves];
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) {
@@ -87,8 +112,8 @@ __device__ __forceinline__ void merge_slots(int fold, const F (*vals)[2 * CAP + 
   if (I == INT_MAX) I = Fmt<F>::max_index;
   const int j = lane < fold ? lane : 0;
   const int64_t b = (int64_t)bits(bin<F>(I + j));
-  int64_t d = 0;
-  F c = F(0);
+  d = 0;
+  c = F(0);
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) {
     const int src = j - (idx[w] - I);  // the value's own slot that lands on slot j
@@ -97,18 +122,13 @@ __device__ __forceinline__ void merge_slots(int fold, const F (*vals)[2 * CAP + 
       c += vals[w][fold + src];
     }
   }
-  constexpr int lo = Lay<F>::mbits - 2;
-  const F pj = fp_from_bits<F>((Ubits)(b + (d & (((int64_t)1 << lo) - 1))));
-  const F cj = c + (F)(d >> lo);
-RB_UNROLL_CAP
-  for (int k = 0; k < CAP; ++k)
-    if (k < fold) {
-      out.p[k] = lane_value(pj, k);
-      out.c[k] = lane_value(cj, k);
-    }
-  if (cls) out.p[0] = class_value<F>(cls);
 }
 
+__device__ __forceinline__ int64_t lane_value_i64(int64_t v, int k) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)(uint64_t)v, k);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)v >> 32), k);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
 // Block length of the reference's driver for a term kind: binnedBLAS_dmdsum
 // and dmddot walk blocks of 2048 and 1024 doubles (dmdsum.ccog:36,
@@ -259,25 +279,44 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
       store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, acc);
     }
   } else if (wave == 0) {
-    // the 4 wave values, slot-parallel in wave 0
-    State<CAP, F> t;
-    merge_slots(fold, lds, lane, t);
-    t.scl = F(0);
-    if (lane == 0) store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, t);
+    // the 4 wave values, slot-parallel in wave 0, then added into the grid
+    // accumulators by thread 0 (a poisoned value contributes its class only)
+    GridAcc* g = reinterpret_cast<GridAcc*>(reinterpret_cast<char*>(a.ticket) + kGridAccOffset);
+    int I, cls;
+    int64_t d;
+    F c;
+    slot_sums<CAP>(fold, lds, lane, I, cls, d, c);
+    if (cls == 0) {
+      const int cp = blockIdx.x % GridAcc::kCopies;
+      for (int k = 0; k < fold; ++k) {
+        const int64_t dk = lane_value_i64(d, k);
+        const int64_t ck = (int64_t)lane_value(c, k);
+        if (lane == 0) {
+          __hip_atomic_fetch_add(&g->d[cp][I + k], dk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(&g->c[cp][I + k], ck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      if (lane == 0)
+        __hip_atomic_fetch_max(&g->top[cp], Fmt<F>::max_index - I, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (lane == 0) {
+      int flags = cls;
+      if (kInfPath) {
+        int64_t fi = lds_fi[0];
+        int big = lds_big[0];
+        for (int w = 1; w < kWaves; ++w) {
+          fi = lds_fi[w] < fi ? lds_fi[w] : fi;
+          big |= lds_big[w];
+        }
+        flags |= big ? 8 : 0;
+        __hip_atomic_fetch_max(&g->fi, INT64_MAX - fi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __hip_atomic_fetch_or(&g->flags, flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   if (threadIdx.x == 0) {
-    if (kInfPath) {
-      int64_t fi = lds_fi[0];
-      int big = lds_big[0];
-      for (int w = 1; w < kWaves; ++w) {
-        fi = lds_fi[w] < fi ? lds_fi[w] : fi;
-        big |= lds_big[w];
-      }
-      a.aux[2 * (int64_t)blockIdx.x] = fi;
-      a.aux[2 * (int64_t)blockIdx.x + 1] = big;
-    }
-    // release the partial (and its record) to the last workgroup; the ticket
-    // itself is relaxed (fence-to-fence synchronisation through it)
+    // release the partial (nrm2) or the accumulator adds to the last
+    // workgroup; the ticket itself is relaxed (fence-to-fence
+    // synchronisation through it)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned ticket = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (ticket == gridDim.x - 1);
@@ -285,94 +324,39 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
   __syncthreads();
   if (!last) return;
 
-  // Last workgroup: merge every partial (exact, so the order is free; it is
-  // fixed anyway), fold in the caller's value, convert.
+  // Last workgroup: the merged value of every workgroup (nrm2: merge the
+  // partial records; the others: read the grid accumulators), then the
+  // caller's value, then convert.
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  State<CAP, F> acc;
-  bool have = false;
-  int64_t fi = INT64_MAX;
-  bool big = false;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
-    State<CAP, F> o;
-    const F* src = a.partials + (int64_t)b * (2 * fold + 1);
-RB_UNROLL_CAP
-    for (int j = 0; j < CAP; ++j)
-      if (j < fold) {
-        o.p[j] = __builtin_nontemporal_load(src + j);
-        o.c[j] = __builtin_nontemporal_load(src + fold + j);
-      }
-    o.scl = __builtin_nontemporal_load(src + 2 * fold);
-    if (kInfPath) {
-      // a finite partial is big when it holds index 0; only a poisoned one
-      // needs its record
-      if (nan_or_inf(o.p[0])) {
-        const int64_t f = a.aux[2 * (int64_t)b];
-        fi = f < fi ? f : fi;
-        big |= a.aux[2 * (int64_t)b + 1] != 0;
-      } else {
-        big |= dmindex0(o.p[0]);
-      }
-    }
-    if (!have) {
-      acc = o;
-      have = true;
-    } else {
-      merge<CAP, T>(fold, acc, o);
-    }
-  }
+  State<CAP, F> acc;  // thread 0: the grid's value
+  int64_t pstar = -1;
   if constexpr (T == kSSq) {
+    bool have = false;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
+      State<CAP, F> o;
+      const F* src = a.partials + (int64_t)b * (2 * fold + 1);
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          o.p[j] = __builtin_nontemporal_load(src + j);
+          o.c[j] = __builtin_nontemporal_load(src + fold + j);
+        }
+      o.scl = __builtin_nontemporal_load(src + 2 * fold);
+      if (!have) {
+        acc = o;
+        have = true;
+      } else {
+        merge<CAP, T>(fold, acc, o);
+      }
+    }
     if (!have) {
       set_bins(fold, Fmt<F>::max_index, acc.p, acc.c);
       acc.scl = 0.0;
     }
     wave_merge_general<CAP, T>(fold, acc);
-  } else {
-    wave_merge_aligned<6>(fold, have, acc);
-  }
-  if (kInfPath) {
-    // only a poisoned value can need the replay: skip the reduction otherwise
-    if (nan_or_inf(uniform_value(acc.p[0]))) fi = wave_min_i64(fi);
-    big = __any(big);
-  }
-  if (lane == 0) {
-    store_state(lds[wave], fold, acc);
-    if (kInfPath) {
-      lds_fi[wave] = fi;
-      lds_big[wave] = big;
-    }
-  }
-  __syncthreads();
-  State<CAP, F> y;
-  if constexpr (T != kSSq) {
-    if (wave == 0) {
-      State<CAP, F> t;
-      merge_slots(fold, lds, lane, t);
-      t.scl = F(0);
-      if (lane == 0) acc = t;
-    }
-  }
-  // Replay decision, made alike by every thread from the values in LDS (no
-  // further barrier): one infinity overall, a finite caller value, and a
-  // magnitude that can overflow a sequential sum.
-  int64_t pstar = -1;
-  if constexpr (kInfPath) {
-    int cls = 0;
-    int64_t f = INT64_MAX;
-    bool bg = false;
-    for (int w = 0; w < kWaves; ++w) {
-      const F p0 = lds[w][0];
-      if (nan_or_inf(p0)) cls |= class_bits(p0);
-      f = lds_fi[w] < f ? lds_fi[w] : f;
-      bg |= lds_big[w] != 0;
-    }
-    const bool one_inf = cls == 2 || cls == 4;
-    const F y0 = a.state_in ? a.state_in[0] : F(0);
-    // (f < n always holds for an infinite class; checked so no replay can
-    // ever index past the vector)
-    if (a.n > 0 && one_inf && f >= 0 && f < a.n && !nan_or_inf(y0) && (bg || dmindex0(y0))) pstar = f;
-  }
-  if (threadIdx.x == 0) {
-    if constexpr (T == kSSq) {
+    if (lane == 0) store_state(lds[wave], fold, acc);
+    __syncthreads();
+    if (threadIdx.x == 0) {
       State<CAP, F> o;
       load_state(lds[0], fold, acc);
       for (int w = 1; w < kWaves; ++w) {
@@ -380,6 +364,72 @@ RB_UNROLL_CAP
         merge<CAP, T>(fold, acc, o);
       }
     }
+  } else {
+    GridAcc* g = reinterpret_cast<GridAcc*>(reinterpret_cast<char*>(a.ticket) + kGridAccOffset);
+    __shared__ int64_t replay_at;
+    if (wave == 0) {
+      // lanes 0..15 hold the copies; the sums over them take the four DPP
+      // stages that stay inside a row of 16 lanes
+      constexpr int kC = GridAcc::kCopies;
+      static_assert(kC == 16, "the copy sums below are the four row-internal stages");
+      const int cl = lane < kC ? lane : 0;
+      const bool own = lane < kC;
+      int top = own ? g->top[cl] : 0;
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const int o = lane_partner32(top, st);
+        top = o > top ? o : top;
+      }
+      top = __builtin_amdgcn_readlane(top, 0);
+      const int I = Fmt<F>::max_index - top;
+      int64_t d[CAP], c[CAP];
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          d[j] = own ? g->d[cl][I + j] : 0;
+          c[j] = own ? g->c[cl][I + j] : 0;
+        }
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) {
+            d[j] += lane_partner_i64(d[j], st);
+            c[j] += lane_partner_i64(c[j], st);
+          }
+      }
+      const int flags = g->flags;
+      const int cls = flags & 7;
+      if (lane == 0) {
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) acc.c[j] = (F)c[j];
+        from_offsets(fold, I, d, acc.p, acc.c);
+        if (cls) acc.p[0] = class_value<F>(cls);
+        acc.scl = F(0);
+      }
+      // Replay decision: one infinity overall, a finite caller value, and a
+      // magnitude that can overflow a sequential sum (a finite term of
+      // index 0 somewhere, or in the caller's value).
+      if constexpr (kInfPath) {
+        const int64_t fim = g->fi;
+        const bool one_inf = cls == 2 || cls == 4;
+        const int64_t f = fim ? INT64_MAX - fim : INT64_MAX;
+        const bool bg = (flags & 8) || (I == 0);
+        const F y0 = a.state_in ? a.state_in[0] : F(0);
+        // (f < n always holds for an infinite class; checked so no replay
+        // can ever index past the vector)
+        if (a.n > 0 && one_inf && f >= 0 && f < a.n && !nan_or_inf(y0) && (bg || dmindex0(y0))) pstar = f;
+      }
+      if (lane == 0) replay_at = pstar;
+    }
+    // wave 0 has read the accumulators: clear them for the next launch
+    __syncthreads();
+    pstar = replay_at;
+    for (int i = threadIdx.x; i < kGridAccWords; i += kThreads) reinterpret_cast<int64_t*>(g)[i] = 0;
+  }
+  State<CAP, F> y;
+  if (threadIdx.x == 0) {
     // caller's value, renormalised first (the reference ends with a renorm)
     if (a.state_in) {
 RB_UNROLL_CAP
@@ -601,8 +651,9 @@ bool launch_seg_term(Term term, const SegArgs& a, int grid, hipStream_t s) {
 
 namespace {
 // ext_scratch (or null for the library workspace): device memory of
-// ext_bytes for the partial and infinity records, and a zero ticket counter
-// that the launch leaves at zero.
+// ext_bytes for the partial records (nrm2), and ext_ticket: a zeroed block of
+// kSyncBytes (the ticket and the grid accumulators) that the launch leaves
+// zeroed.
 template <class F>
 bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const F* y, int64_t incy,
                 const F* state_in, F scale_in, const F* scale_in_dev, F* state_out, F* scale_out, F* result,
@@ -644,9 +695,9 @@ bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const 
   const int64_t cap = (int64_t)dev->cus * (mode == 2 ? kBlocksPerCUStrided : blocks_per_cu<F>());
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  // partial records, then (8-byte aligned) the PoisonInfo records
-  const size_t rec_bytes = (sizeof(F) * (size_t)blocks * (2 * fold + 1) + 7) & ~(size_t)7;
-  const size_t need = rec_bytes + sizeof(int64_t) * 2 * (size_t)blocks;
+  // nrm2: one partial record per workgroup; the others merge through the
+  // grid accumulators in the ticket block and need no records
+  const size_t need = term == kSSq ? sizeof(F) * (size_t)blocks * (2 * fold + 1) : 0;
   char* ws;
   if (ext_scratch) {
     if (need > ext_bytes) {
@@ -659,11 +710,10 @@ bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const 
     ws = (char*)ext_scratch;
     a.ticket = ext_ticket;
   } else {
-    ws = (char*)workspace(s, kSlotPartials, need);
-    a.ticket = tickets(s, 1);
+    ws = (char*)workspace(s, kSlotPartials, need > 0 ? need : 1);
+    a.ticket = tickets(s, (int)((kSyncBytes + sizeof(unsigned) - 1) / sizeof(unsigned)));
   }
   a.partials = (F*)ws;
-  a.aux = ws ? (int64_t*)(ws + rec_bytes) : nullptr;
   if (!a.partials || !a.ticket) return false;
   switch (term) {
     case kSum: return blas1_launch_sum(mode, a, (int)blocks, s);

@@ -26,9 +26,11 @@ bool blas1(int fold, Term term, int64_t n, const double* x, int64_t incx, const 
            double* scale_out, double* result, hipStream_t s);
 // The converted double result of a BLAS1 reduction with the caller's scratch
 // instead of the library workspace: scratch holds scratch_bytes of device
-// memory for the workgroup records (the launch fails with kBadArgument when
-// the grid needs more), ticket a device counter that is zero on entry and
-// that the launch leaves at zero. Allocates nothing.
+// memory for the workgroup records of nrm2 (the launch fails with
+// kBadArgument when the grid needs more; the other terms need none), ticket
+// a zeroed device block of kBlas1SyncBytes (completion ticket and grid
+// accumulators) that the launch leaves zeroed. Allocates nothing.
+constexpr size_t kBlas1SyncBytes = 32768;
 bool blas1_scratch(int fold, Term term, int64_t n, const double* x, int64_t incx, const double* y, int64_t incy,
                    double* result, void* scratch, size_t scratch_bytes, unsigned* ticket, hipStream_t s);
 // The same for float operands and float_binned values (folds 2..21).

@@ -1,7 +1,7 @@
 // noalloc.hip -- reproblas_hip_rddot_device_noalloc: reproblas_hip_rddot_device
-// with its scratch (the workgroup partial records, their infinity records
-// and the completion ticket) in a fixed device array of this file instead of
-// the library's grown-on-demand workspace, for callers that must not
+// with its scratch (the completion ticket and the grid accumulators through
+// which the workgroups merge) in a fixed device array of this file instead
+// of the library's grown-on-demand workspace, for callers that must not
 // allocate (code under hipGraph capture, tight serving loops).
 //
 // The array lives in the code object: every GPU the library runs on holds
@@ -19,16 +19,18 @@
 namespace rb {
 namespace {
 
-// 512 KiB: the partial records of 256 workgroups at any fold (unit stride
-// runs one workgroup per CU on 256 CUs) or of 1024 workgroups up to fold 30
-// (strided operands run four per CU), their infinity records, and the ticket
-// in the last 256 bytes.
-constexpr size_t kScratchBytes = 512 * 1024;
-constexpr size_t kRecordBytes = kScratchBytes - 256;
+// 64 KiB: the 32 KiB sync block (ticket and grid accumulators, which the kernels
+// leave zeroed) in the last kBlas1SyncBytes; the rest would hold workgroup
+// records, which the dot does not use (it merges through the accumulators
+// whatever the grid), so the dot needs no more than this at any size.
+constexpr size_t kScratchBytes = 64 * 1024;
+constexpr size_t kRecordBytes = kScratchBytes - kBlas1SyncBytes;
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
-__global__ void clear_ticket_kernel(unsigned* ticket) { *ticket = 0u; }
+__global__ void clear_sync_kernel(unsigned* sync) {
+  for (size_t i = threadIdx.x; i < kBlas1SyncBytes / sizeof(unsigned); i += blockDim.x) sync[i] = 0u;
+}
 
 struct DeviceScratch {
   unsigned char* ptr = nullptr;  // device address of `scratch` on that GPU
@@ -54,9 +56,9 @@ unsigned char* device_scratch(hipStream_t s) {
       set_error(kHipError, "hipGetSymbolAddress returned no address for the scratch array");
       return nullptr;
     }
-    // the kernels leave the ticket at zero; start it there (a kernel, not a
-    // memset, so the call may also be captured into a graph)
-    clear_ticket_kernel<<<1, 1, 0, s>>>((unsigned*)((unsigned char*)p + kRecordBytes));
+    // the kernels leave the sync block zeroed; start it there (a kernel, not
+    // a memset, so the call may also be captured into a graph)
+    clear_sync_kernel<<<1, 256, 0, s>>>((unsigned*)((unsigned char*)p + kRecordBytes));
     if (!RB_HIP(hipGetLastError())) return nullptr;
     d.ptr = (unsigned char*)p;
   }

@@ -1,7 +1,7 @@
 """reproblas_hip_rddot_device_noalloc: the device dot with its scratch in the
 library's fixed device array. Two calls in a row (the second smaller) and
 mixed strides give the same bits as reproblas_hip_rddot_device, which the
-other tests pin to the oracle; sizes beyond the scratch are refused."""
+other tests pin to the oracle; bad arguments are refused."""
 import ctypes
 
 import numpy as np
@@ -55,7 +55,7 @@ def test_noalloc_dot_matches_allocating_entry_point(gpu):
 
 def test_noalloc_dot_infinity_replay(gpu):
     """The infinity-path replay runs inside the same launch (last workgroup),
-    so it needs no scratch beyond the partials."""
+    so it needs no scratch beyond the fixed sync block."""
     import torch
 
     import reproblas_amd as rb
@@ -85,11 +85,15 @@ def test_noalloc_refuses_bad_arguments(gpu):
     tx = torch.zeros(16, dtype=torch.float64, device=gpu)
     out = torch.zeros(1, dtype=torch.float64, device=gpu)
     assert f(1, 16, dp(tx), 1, dp(tx), 1, dp(out), s) != 0  # fold outside [2, 52]
-    # strided operands use four workgroups per CU: fold 52 needs more partial
-    # records than the fixed scratch holds, and the call says so
-    big = torch.zeros(1 << 24, dtype=torch.float64, device=gpu)
-    assert f(52, (1 << 23) - 1, dp(big), 2, dp(big), 2, dp(out), s) != 0
-    assert b"scratch" in L.reproblas_hip_last_error_string()
+    # strided operands use four workgroups per CU; at fold 52 that grid still
+    # merges through the fixed sync block (no per-workgroup records), so the
+    # call succeeds with the allocating entry point's bits
+    big = torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, 1 << 24)).to(gpu)
+    two = torch.zeros(2, dtype=torch.float64, device=gpu)
+    assert f(52, (1 << 23) - 1, dp(big), 2, dp(big, 1), 2, dp(two, 1), s) == 0
+    assert L.reproblas_hip_rddot_device(52, (1 << 23) - 1, dp(big), 2, dp(big, 1), 2, dp(two, 0), s) == 0
+    got = two.cpu().numpy()
+    assert got.view(np.uint64)[0] == got.view(np.uint64)[1], got
     host = np.zeros(16)
     assert f(3, 16, ctypes.c_void_p(host.ctypes.data), 1, dp(tx), 1, dp(out), s) != 0  # host memory
     L.reproblas_hip_clear_error()
