@@ -26,6 +26,8 @@
 // recomputes that output's class from its K terms (NaN, +inf or -inf).
 #include <limits.h>
 
+#include <algorithm>
+
 #include "binned_core.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -103,6 +105,43 @@ __global__ __launch_bounds__(256) void chunk_max_kernel(int64_t rows, int64_t K,
   F m = F(0);
   for (int64_t k = k0; k < k1; ++k) m = fmax(m, fabs(X[r * s_row + k * s_k]));
   out[ch * rows + r] = m;
+}
+
+// The same maxima for an operand whose k index is contiguous (s_k == 1):
+// each half-wave reads one chunk of a row (32 consecutive elements, one per
+// lane; KC == 32) and reduces it across its lanes, so the reads coalesce.
+// (Thread-per-(row, chunk) reads would be a whole leading dimension apart.)
+static_assert(KC == 32, "one chunk per half-wave");
+template <class F>
+__global__ __launch_bounds__(256) void chunk_max_kcontig_kernel(int64_t rows, int64_t K, const F* X, int64_t s_row,
+                                                                F* out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t chunks = (K + KC - 1) / KC;
+  const int64_t segs = (chunks + 1) / 2;  // 64-element segments per row
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; w < rows * segs; w += nw) {
+    const int64_t r = w / segs, sg = w - r * segs;
+    const int64_t k = sg * 64 + lane;
+    F m = k < K ? fabs(X[r * s_row + k]) : F(0);
+#pragma unroll
+    for (int off = 1; off < 32; off *= 2) m = fmax(m, __shfl_xor(m, off, 64));
+    m = fmax(F(0), m);  // as the running max from 0 of chunk_max_kernel
+    const int64_t ch = 2 * sg + (lane >> 5);
+    if ((lane & 31) == 0 && ch < chunks) out[ch * rows + r] = m;
+  }
+}
+
+// Chunk maxima of one operand (rows x K, elements s_row / s_k apart).
+template <class F>
+void launch_chunk_max(int64_t rows, int64_t K, const F* X, int64_t s_row, int64_t s_k, F* out, hipStream_t s) {
+  if (s_k == 1) {
+    const int64_t segs = rows * ((K + 2 * KC - 1) / (2 * KC));
+    const int64_t blocks = std::min<int64_t>((segs + 3) / 4, 65536);
+    chunk_max_kcontig_kernel<<<(unsigned)blocks, 256, 0, s>>>(rows, K, X, s_row, out);
+  } else {
+    const int64_t chunks = (K + KC - 1) / KC;
+    chunk_max_kernel<<<(unsigned)((rows * chunks + 255) / 256), 256, 0, s>>>(rows, K, X, s_row, s_k, out);
+  }
 }
 
 template <int CAP, class F>
@@ -426,9 +465,8 @@ bool gemm_common(const GemmDescT<F>& g0, hipStream_t s) {
   if (!mx) return false;
   g.rowmax = mx;
   g.colmax = mx + chunks * g.M;
-  chunk_max_kernel<<<(unsigned)((g.M * chunks + 255) / 256), 256, 0, s>>>(g.M, g.K, g.A, g.sai, g.sak, mx);
-  chunk_max_kernel<<<(unsigned)((g.N * chunks + 255) / 256), 256, 0, s>>>(g.N, g.K, g.B, g.sbj, g.sbk,
-                                                                          mx + chunks * g.M);
+  launch_chunk_max(g.M, g.K, g.A, g.sai, g.sak, mx, s);
+  launch_chunk_max(g.N, g.K, g.B, g.sbj, g.sbk, mx + chunks * g.M, s);
   if (!RB_HIP(hipGetLastError())) return false;
   const bool ai = (g.sai == 1), bk = (g.sbk == 1);
   switch (fold) {

@@ -460,11 +460,12 @@ struct WaveRun {
   int dep;  // deposits since the last renormalisation
 };
 
-// Deposits one tile of raw terms (cx, cy) into the lane's bins. The even
-// terms go into the lane's primaries and the odd ones into a second set that
-// starts the tile at the same bins, so the two dependency chains interleave;
-// the second set's offsets then join the first (exact: together they hold
-// exactly the slices one set would, within the endurance).
+// Deposits one tile of raw terms (cx, cy) into the lane's bins. In tiles of
+// 8 or more terms the even terms go into the lane's primaries and the odd
+// ones into a second set that starts the tile at the same bins, so the two
+// dependency chains interleave; the second set's offsets then join the first
+// (exact: together they hold exactly the slices one set would, within the
+// endurance).
 template <int CAP, int K, int U, class F>
 __device__ __forceinline__ void deposit_tile(int fold, const F (&cx)[U], const F (&cy)[U], WaveRun<F>& w,
                                              State<CAP, F>& s) {
@@ -519,10 +520,6 @@ __device__ __forceinline__ void deposit_tile(int fold, const F (&cx)[U], const F
     }
     w.lim = admit_limit<F>(w.idx);
   }
-  F q[CAP];
-RB_UNROLL_CAP
-  for (int j = 0; j < CAP; ++j)
-    if (j < fold) q[j] = bin<F>(w.idx + j);
   if (K == kSSq) {
 #pragma unroll
     for (int k = 0; k < U; ++k) {
@@ -531,18 +528,33 @@ RB_UNROLL_CAP
     }
   }
   const bool top = (w.idx == 0);
-  if (top) {
-#pragma unroll
-    for (int k = 0; k + 1 < U; k += 2) deposit2(fold, true, v[k], s.p, v[k + 1], q);
-    if (U % 2) deposit(fold, true, v[U - 1], s.p);
-  } else {
-#pragma unroll
-    for (int k = 0; k + 1 < U; k += 2) deposit2(fold, false, v[k], s.p, v[k + 1], q);
-    if (U % 2) deposit(fold, false, v[U - 1], s.p);
-  }
+  if constexpr (U >= 8) {
+    F q[CAP];
 RB_UNROLL_CAP
-  for (int j = 0; j < CAP; ++j)
-    if (j < fold) s.p[j] = s.p[j] + (q[j] - bin<F>(w.idx + j));
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) q[j] = bin<F>(w.idx + j);
+    if (top) {
+#pragma unroll
+      for (int k = 0; k + 1 < U; k += 2) deposit2(fold, true, v[k], s.p, v[k + 1], q);
+      if (U % 2) deposit(fold, true, v[U - 1], s.p);
+    } else {
+#pragma unroll
+      for (int k = 0; k + 1 < U; k += 2) deposit2(fold, false, v[k], s.p, v[k + 1], q);
+      if (U % 2) deposit(fold, false, v[U - 1], s.p);
+    }
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) s.p[j] = s.p[j] + (q[j] - bin<F>(w.idx + j));
+  } else {
+    // short tiles: one chain (the second set costs more than it overlaps)
+    if (top) {
+#pragma unroll
+      for (int k = 0; k < U; ++k) deposit(fold, true, v[k], s.p);
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k) deposit(fold, false, v[k], s.p);
+    }
+  }
   w.dep += U;
 }
 
