@@ -273,13 +273,18 @@ bool zgemv_into(int fold, char order, char trans, int64_t M, int64_t N, const F*
   if (M <= 0 || N <= 0 || (alpha[0] == F(0) && alpha[1] == F(0))) return true;
   const bool rm = row_major(order);
   const int tc = trans_code(trans);
-  const int64_t Mo = tc == 0 ? M : N, No = tc == 0 ? N : M;
+  const int64_t Mo = tc == 0 ? M : N, No_all = tc == 0 ? N : M;
   const int mode = (alpha[0] == F(1) && alpha[1] == F(0)) ? 0 : 1;
   const int conj = (tc == 2) ? 1 : 0;
   // rows of op(A) are contiguous for row-major 'N' and column-major 'T'/'C':
   // two real row-dot passes over the interleaved rows; otherwise the terms of
   // an output are lda apart: one complex column pass
   const bool rows_contig = rm ? (tc == 0) : (tc != 0);
+  // column-major conjugate transpose: the reference dots MIN(X_BLOCK, N - j)
+  // rows of each M-row column per block j < M (zbzgemv.ccog:214-220,
+  // cbcgemv.ccog the same), i.e. the first N rows when M > N; for M < N it
+  // reads past the column (no defined answer) and the M rows are used
+  const int64_t No = (!rm && tc == 2 && M > N) ? N : No_all;
   F* xs = (F*)workspace(s, kSlotZB, sizeof(F) * (size_t)(2 * No));
   if (!xs) return false;
   alpha_x_kernel<F><<<grid1(No), kT, 0, s>>>(No, x, incx, alpha[0], alpha[1], mode, xs);

@@ -97,3 +97,32 @@ def test_nrm2_wide_range(gpu, seed, fold):
         assert len(keep) == 2 * fold
     assert np.array_equal(Yg[keep].view(np.uint64), Ye[keep].view(np.uint64)), (seed, fold)
     assert tc.same(O.ddbconv(fold, Yg), O.ddbconv(fold, Ye))
+
+
+@pytest.mark.parametrize("name,x,y", tc.AMAX_CASES, ids=[c[0] for c in tc.AMAX_CASES])
+def test_amax_nan_semantics(gpu, name, x, y):
+    """binnedBLAS_damax / damaxm / samax / samaxm with NaN terms: the
+    reference's scalar answer (max after the last NaN term, or NaN when it is
+    last), host and device operands, against the oracle."""
+    import torch
+
+    import reproblas_amd as rb
+
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    n = len(x)
+    O, Os = oracle(), oracle_s()
+    e = O.fn["damaxm"](n, vp(x), 1, vp(y), 1) if y is not None else O.fn["damax"](n, vp(x), 1)
+    xs = x.astype(np.float32)
+    ys = y.astype(np.float32) if y is not None else None
+    es = Os.fn["samaxm"](n, vp(xs), 1, vp(ys), 1) if ys is not None else Os.fn["samax"](n, vp(xs), 1)
+    for dev in (None, gpu):
+        tx = x if dev is None else torch.from_numpy(x).to(dev)
+        txs = xs if dev is None else torch.from_numpy(xs).to(dev)
+        if y is None:
+            g, gs = rb.damax(tx), rb.samax(txs)
+        else:
+            ty = y if dev is None else torch.from_numpy(y).to(dev)
+            tys = ys if dev is None else torch.from_numpy(ys).to(dev)
+            g, gs = rb.damaxm(tx, ty), rb.samaxm(txs, tys)
+        assert tc.same(g, e), (name, dev, g, e)
+        assert tc.same32(gs, es), (name, dev, gs, es)

@@ -245,3 +245,75 @@ def test_ssq_slots_beyond_bin_table_depend_on_the_split(fold):
             assert same(L.ddbconv(fold, Y), L.ddbconv(fold, B))
             differs += not np.array_equal(Y.view(np.uint64), B.view(np.uint64))
     assert differs > 0  # the slots past the table do depend on the split
+
+
+# ------------------------------------------------------------ amax and NaN --
+def amax_nan_cases():
+    """(name, x, y or None) with NaN terms at chosen positions: in the middle
+    (the max after it counts), last (the result is NaN), several (the last
+    one decides), and a NaN product from 0 * inf (damaxm)."""
+    g = np.random.default_rng(4242)
+    out = []
+    for n in (1, 2, 7, 2048, 2049, 100003):
+        for pos in sorted({0, n // 3, n - 2, n - 1} - {-1}):
+            if pos < 0:
+                continue
+            x = g.uniform(-4, 4, n)
+            x[pos] = np.nan
+            if pos + 1 < n:
+                x[min(n - 1, pos + 1 + (n - pos) // 2)] = -7.5  # a large term after the NaN
+            out.append((f"x_n{n}_nan{pos}", x, None))
+    x = g.uniform(-1, 1, 5000)
+    x[[10, 2000, 4321]] = np.nan
+    x[5] = 99.0  # before the last NaN: dropped
+    out.append(("x_three_nans", x, None))
+    x = g.uniform(-1, 1, 3000)
+    y = g.uniform(-1, 1, 3000)
+    x[1500], y[1500] = 0.0, np.inf  # NaN product
+    x[1499] = 50.0
+    out.append(("xy_zero_times_inf", x, y))
+    x = g.uniform(-1, 1, 3000)
+    y = g.uniform(-1, 1, 3000)
+    x[2999], y[2999] = np.inf, 0.0  # NaN product last
+    out.append(("xy_last_nan", x, y))
+    return out
+
+
+AMAX_CASES = amax_nan_cases()
+
+
+def amax_expected(x, y):
+    """The reference's scalar loop restated: max |t| after the last NaN term,
+    or NaN when that term is last (damax.ccog:282-300)."""
+    with np.errstate(invalid="ignore"):
+        t = np.abs(x * y if y is not None else x)
+    nan = np.flatnonzero(np.isnan(t))
+    if len(nan) == 0:
+        return float(max(0.0, t.max()))
+    j = nan[-1]
+    if j == len(t) - 1:
+        return float("nan")
+    return float(t[j + 1:].max())
+
+
+@pytest.mark.parametrize("name,x,y", AMAX_CASES, ids=[c[0] for c in AMAX_CASES])
+def test_amax_nan_semantics(name, x, y):
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    O = oracle()
+    n = len(x)
+    got = O.fn["damaxm"](n, vp(x), 1, vp(y), 1) if y is not None else O.fn["damax"](n, vp(x), 1)
+    assert same(got, amax_expected(x, y)), (name, got)
+    R = reference("sisd")
+    if R is not None:
+        r = R.fn["damaxm"](n, vp(x), 1, vp(y), 1) if y is not None else R.fn["damax"](n, vp(x), 1)
+        assert same(r, got), (name, r, got)
+    # float path: the same positions
+    xs = x.astype(np.float32)
+    ys = y.astype(np.float32) if y is not None else None
+    Os = oracle_s()
+    gs = Os.fn["samaxm"](n, vp(xs), 1, vp(ys), 1) if ys is not None else Os.fn["samax"](n, vp(xs), 1)
+    assert same32(gs, np.float32(amax_expected(xs, ys))), (name, gs)
+    Rs = reference_s("sisd")
+    if Rs is not None:
+        rs = Rs.fn["samaxm"](n, vp(xs), 1, vp(ys), 1) if ys is not None else Rs.fn["samax"](n, vp(xs), 1)
+        assert same32(rs, gs), (name, rs, gs)
