@@ -62,6 +62,7 @@ struct Blas1Args {
   const F* y;
   int64_t incy;
   F* partials;    // nrm2: gridDim.x records of (2*fold+1) elements
+  F* mins;        // nrm2: gridDim.x smallest nonzero |x| of each workgroup
   unsigned* ticket;  // zeroed block of kSyncBytes: the ticket, then a GridAcc
   const F* state_in;
   F scale_in;
@@ -144,13 +145,15 @@ constexpr int ref_block() {
 
 // The binned sum of the terms [lo, hi) by the 4 waves of one workgroup;
 // thread 0 ends with the merged value (renormalised). Every thread calls it.
+// nrm2: every term is squared at scale0, which must be at least dscale of
+// every |x| in the range (no wave changes scale).
 template <int CAP, int T, int U, int MODE, class F>
 __device__ void wg_range_sum(const Blas1Args<F>& a, int fold, int64_t lo, int64_t hi, int wave, int lane,
-                             F (*lds)[2 * CAP + 1], State<CAP, F>& out) {
+                             F (*lds)[2 * CAP + 1], State<CAP, F>& out, F scale0 = F(0)) {
   const Span<F> sp = make_span<U>(a.x + lo * a.incx, a.incx, T == kDot ? a.y + lo * a.incy : nullptr, a.incy,
                                   hi - lo);
   State<CAP, F> st;
-  wave_accumulate<CAP, T, U, MODE>(sp, fold, wave, kWaves, lane, st, nullptr);
+  wave_accumulate<CAP, T, U, MODE>(sp, fold, wave, kWaves, lane, st, nullptr, scale0);
   __syncthreads();
   if (lane == 0) store_state(lds[wave], fold, st);
   __syncthreads();
@@ -239,6 +242,95 @@ RB_UNROLL_CAP
   return p0;
 }
 
+// The reference's sum of squares in one call (dmdssq.ccog:60-101): each
+// block of ref_block() terms squares at the running scale of the block
+// maxima so far (from the caller's scaleY), and the accumulator is rescaled
+// (dmdrescale) whenever that scale grows. Its value is therefore the ordered
+// merge of "epochs" -- maximal runs of blocks with one scale -- each summed
+// exactly at its scale, with the accumulator rescaled at every epoch start.
+// The result equals the parallel one unless a scaled square is subnormal and
+// the value's lowest bin is fine enough to hold part of it (bin 51 for
+// double, 19-20 for float); the last workgroup then replays this schedule. Called by every thread of the last
+// workgroup; thread 0 ends with the value (y) and its scale (y.scl).
+template <int CAP, int U, int MODE, class F>
+__device__ void replay_ssq(const Blas1Args<F>& a, int fold, int wave, int lane, F (*lds)[2 * CAP + 1],
+                           State<CAP, F>& y) {
+  constexpr int64_t NB = ref_block<kSSq, F>();
+  constexpr int kMaxEpochs = 96;  // scales are distinct powers 2^(WIDTH*k+c): far fewer
+  const int64_t nb = (a.n + NB - 1) / NB;
+  const F scale_in = a.scale_in_dev ? *a.scale_in_dev : a.scale_in;
+  __shared__ F blk_scale[kWaves];
+  __shared__ int64_t ep_start[kMaxEpochs];
+  __shared__ F ep_scale[kMaxEpochs];
+  __shared__ int nep;
+  __shared__ F run;
+  if (threadIdx.x == 0) {
+    nep = 0;
+    run = scale_in;
+  }
+  __syncthreads();
+  // pass 1: dscale(max |x|) of every block in order, epoch starts where the
+  // running scale grows (block 0 always starts one)
+  for (int64_t g = 0; g < nb; g += kWaves) {
+    const int64_t b = g + wave;
+    if (b < nb) {
+      const int64_t e1 = (b + 1) * NB < a.n ? (b + 1) * NB : a.n;
+      F m = F(0);
+      for (int64_t e = b * NB + lane; e < e1; e += 64) m = fmax(m, fabs(a.x[e * a.incx]));
+      m = wave_max(m);
+      if (lane == 0) blk_scale[wave] = dscale(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 0; w < kWaves && g + w < nb; ++w) {
+        const F sc = blk_scale[w];
+        if (g + w == 0 || sc > run) {
+          run = sc > run ? sc : run;
+          if (nep < kMaxEpochs) {
+            ep_start[nep] = g + w;
+            ep_scale[nep] = run;
+          }
+          ++nep;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // pass 2: each epoch summed at its scale, merged in order into the
+  // caller's value as given
+  if (threadIdx.x == 0) {
+    if (a.state_in) {
+RB_UNROLL_CAP
+      for (int j = 0; j < CAP; ++j)
+        if (j < fold) {
+          y.p[j] = a.state_in[j];
+          y.c[j] = a.state_in[fold + j];
+        }
+    } else {
+      set_zero(fold, y.p, y.c);
+    }
+    y.scl = scale_in;
+  }
+  const int ne = nep < kMaxEpochs ? nep : kMaxEpochs;
+  F scl = scale_in;
+  for (int e = 0; e < ne; ++e) {
+    const int64_t lo = ep_start[e] * NB;
+    const int64_t hi = (e + 1 < ne) ? ep_start[e + 1] * NB : a.n;
+    const F sc = ep_scale[e];
+    State<CAP, F> P;
+    wg_range_sum<CAP, kSSq, U, MODE>(a, fold, lo, hi, wave, lane, lds, P, sc);
+    if (threadIdx.x == 0) {
+      // the caller's value is rescaled as given; later the accumulator was
+      // renormalised before each block, so before each rescale
+      if (sc > scl && scl > F(0)) rescale(fold, sc, scl, y.p, y.c);
+      scl = sc;
+      renorm(fold, y.p, y.c);
+      add(fold, y.p, y.c, P.p, P.c);
+      y.scl = scl;
+    }
+  }
+}
+
 template <int CAP, bool FIXED, int T, int U, int MODE, class F>
 __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
   constexpr bool kInfPath = (T == kSum || T == kDot);
@@ -252,15 +344,19 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
   PoisonInfo pi;
   // nrm2: every wave squares at no less than the caller's scale
   const F scale0 = (T == kSSq && a.state_in) ? (a.scale_in_dev ? *a.scale_in_dev : a.scale_in) : F(0);
-  wave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s, kInfPath ? &pi : nullptr, scale0);
+  F wmin = (F)__builtin_inf();  // nrm2: smallest nonzero |x| of the wave
+  wave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s, kInfPath ? &pi : nullptr, scale0,
+                                   T == kSSq ? &wmin : nullptr);
 
   // Workgroup merge through LDS, then one partial per workgroup.
   __shared__ F lds[kWaves][2 * CAP + 1];
   __shared__ int64_t lds_fi[kWaves];
   __shared__ int lds_big[kWaves];
+  __shared__ F lds_min[kWaves];
   __shared__ bool last;
   if (lane == 0) {
     store_state(lds[wave], fold, s);
+    if (T == kSSq) lds_min[wave] = wmin;
     if (kInfPath) {
       lds_fi[wave] = pi.first_inf;
       lds_big[wave] = pi.big;
@@ -272,11 +368,14 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
     if (threadIdx.x == 0) {
       State<CAP, F> acc, o;
       load_state(lds[0], fold, acc);
+      F mn = lds_min[0];
       for (int w = 1; w < kWaves; ++w) {
         load_state(lds[w], fold, o);
         merge<CAP, T>(fold, acc, o);
+        mn = lds_min[w] < mn ? lds_min[w] : mn;
       }
       store_state(a.partials + (int64_t)blockIdx.x * (2 * fold + 1), fold, acc);
+      a.mins[blockIdx.x] = mn;
     }
   } else if (wave == 0) {
     // the 4 wave values, slot-parallel in wave 0, then added into the grid
@@ -330,9 +429,13 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   State<CAP, F> acc;  // thread 0: the grid's value
   int64_t pstar = -1;
+  F gmin = (F)__builtin_inf();  // nrm2, thread 0: smallest nonzero |x| of the grid
   if constexpr (T == kSSq) {
     bool have = false;
+    F mn = (F)__builtin_inf();
     for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
+      const F bm = __builtin_nontemporal_load(a.mins + b);
+      mn = bm < mn ? bm : mn;
       State<CAP, F> o;
       const F* src = a.partials + (int64_t)b * (2 * fold + 1);
 RB_UNROLL_CAP
@@ -354,14 +457,24 @@ RB_UNROLL_CAP
       acc.scl = 0.0;
     }
     wave_merge_general<CAP, T>(fold, acc);
-    if (lane == 0) store_state(lds[wave], fold, acc);
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+      const F o = lane_partner(mn, st);
+      mn = o < mn ? o : mn;
+    }
+    if (lane == 0) {
+      store_state(lds[wave], fold, acc);
+      lds_min[wave] = mn;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
       State<CAP, F> o;
       load_state(lds[0], fold, acc);
+      gmin = lds_min[0];
       for (int w = 1; w < kWaves; ++w) {
         load_state(lds[w], fold, o);
         merge<CAP, T>(fold, acc, o);
+        gmin = lds_min[w] < gmin ? lds_min[w] : gmin;
       }
     }
   } else {
@@ -469,6 +582,29 @@ RB_UNROLL_CAP
     }
     // n == 0: nothing deposited, the caller's value only (the reference's final renorm)
     renorm(fold, y.p, y.c);
+  }
+  if constexpr (T == kSSq) {
+    // The reference's one-call schedule where it can differ (replay_ssq):
+    // a finite value whose lowest bin is fine enough to see a subnormal
+    // square (its ulp below twice the smallest normal: bin 51 for double,
+    // bins 19-20 for float) and a term whose square at the final scale is
+    // subnormal (|x| / scale < 2^-(MAX_EXP/2 - 1); the reference's earlier
+    // block scales are no larger, so its squares are no smaller).
+    __shared__ int ssq_replay;
+    if (threadIdx.x == 0) {
+      const bool fin = !nan_or_inf(y.p[0]) && y.p[0] != F(0);
+      const F tiny = fp_from_bits<F>((typename Fmt<F>::U)(Lay<F>::ieee_bias - (Fmt<F>::max_exp / 2 - 1))
+                                     << Lay<F>::mbits);
+      const bool fine = fin && bin_exp<F>(dmindex(y.p[0]) + fold - 1) < Lay<F>::mbits + 2;
+      ssq_replay = a.n > 0 && fine && y.scl > F(0) && gmin / y.scl < tiny;
+    }
+    __syncthreads();
+    if (ssq_replay) {
+      replay_ssq<CAP, U, MODE>(a, fold, wave, lane, lds, y);
+      if (threadIdx.x == 0) renorm(fold, y.p, y.c);
+    }
+  }
+  if (threadIdx.x == 0) {
     F scale = y.scl;
     if (T == kSSq && nan_or_inf(y.p[0])) scale = F(1);  // dscale(1.0), dmdssq.ccog tail
     if (a.state_out) {
@@ -695,9 +831,10 @@ bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const 
   const int64_t cap = (int64_t)dev->cus * (mode == 2 ? kBlocksPerCUStrided : blocks_per_cu<F>());
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  // nrm2: one partial record per workgroup; the others merge through the
-  // grid accumulators in the ticket block and need no records
-  const size_t need = term == kSSq ? sizeof(F) * (size_t)blocks * (2 * fold + 1) : 0;
+  // nrm2: one partial record and one minimum per workgroup; the others
+  // merge through the grid accumulators in the ticket block and need none
+  const size_t rec_bytes = sizeof(F) * (size_t)blocks * (2 * fold + 1);
+  const size_t need = term == kSSq ? rec_bytes + sizeof(F) * (size_t)blocks : 0;
   char* ws;
   if (ext_scratch) {
     if (need > ext_bytes) {
@@ -714,6 +851,7 @@ bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const 
     a.ticket = tickets(s, (int)((kSyncBytes + sizeof(unsigned) - 1) / sizeof(unsigned)));
   }
   a.partials = (F*)ws;
+  a.mins = ws ? (F*)(ws + (term == kSSq ? rec_bytes : 0)) : nullptr;
   if (!a.partials || !a.ticket) return false;
   switch (term) {
     case kSum: return blas1_launch_sum(mode, a, (int)blocks, s);

@@ -457,6 +457,7 @@ struct WaveRun {
   F lim;    // admit_limit(idx)
   F scl;    // nrm2: current scale of the wave
   F inv;    // nrm2: 1 / scl
+  F mn;     // nrm2: smallest nonzero |x| the lane has seen (+inf: none)
   int dep;  // deposits since the last renormalisation
 };
 
@@ -492,6 +493,15 @@ __device__ __forceinline__ void deposit_tile(int fold, const F (&cx)[U], const F
     need = (dscale(m) > w.scl) || (ms * ms >= w.lim);
   } else {
     need = (m >= w.lim);
+  }
+  if (K == kSSq) {
+    F mk = (F)__builtin_inf();
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const F ak = fabs(v[k]);  // the raw term (squared further down)
+      mk = (ak > F(0) && ak < mk) ? ak : mk;
+    }
+    w.mn = mk < w.mn ? mk : w.mn;
   }
   // the endurance renorm comes before the update as in the reference drivers
   if (w.dep + U > Fmt<F>::endurance) {
@@ -570,6 +580,7 @@ __device__ __forceinline__ void wave_start(int fold, WaveRun<F>& w, State<CAP, F
   w.lim = admit_limit<F>(w.idx);
   w.scl = scale0;
   w.inv = scale0 > F(0) ? F(1) / scale0 : F(0);  // exact: scales are powers of two
+  w.mn = (F)__builtin_inf();
   w.dep = 0;
 }
 
@@ -612,10 +623,12 @@ __device__ __forceinline__ void wave_finish(const Span<F>& a, int fold, int64_t 
 
 // One wave reduces the tiles t0, t0 + tstep, ... of a span into `s` (every
 // lane ends with the wave's value; a poisoned value is replaced by the class
-// of the terms). `info` (or null) receives what the infinity-path replay needs.
+// of the terms). `info` (or null) receives what the infinity-path replay needs;
+// `min_nz` (or null; nrm2) the smallest nonzero |x| of the wave.
 template <int CAP, int K, int U, int MODE, class F>
 __device__ __forceinline__ void wave_accumulate(const Span<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                                State<CAP, F>& s, PoisonInfo* info = nullptr, F scale0 = F(0)) {
+                                                State<CAP, F>& s, PoisonInfo* info = nullptr, F scale0 = F(0),
+                                                F* min_nz = nullptr) {
   WaveRun<F> w;
   wave_start(fold, w, s, scale0);
 
@@ -636,6 +649,16 @@ __device__ __forceinline__ void wave_accumulate(const Span<F>& a, int fold, int6
     t = tn;
   }
   wave_finish<CAP, K, U, MODE>(a, fold, t0, tstep, lane, w, s, info);
+  if (min_nz) {
+    // smallest nonzero |x| of the wave (nrm2), every lane
+    F m = w.mn;
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+      const F o = lane_partner(m, st);
+      m = o < m ? o : m;
+    }
+    *min_nz = m;
+  }
 }
 
 }  // namespace

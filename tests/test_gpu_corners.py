@@ -126,3 +126,30 @@ def test_amax_nan_semantics(gpu, name, x, y):
             g, gs = rb.damaxm(tx, ty), rb.samaxm(txs, tys)
         assert tc.same(g, e), (name, dev, g, e)
         assert tc.same32(gs, es), (name, dev, gs, es)
+
+
+@pytest.mark.parametrize("dt,seed,fold", tc.SSQ_SUB_CASES,
+                         ids=[f"{'d' if c[0] == np.float64 else 's'}{c[1]}_f{c[2]}" for c in tc.SSQ_SUB_CASES])
+def test_ssq_subnormal_squares(gpu, dt, seed, fold):
+    """dbdssq / sbsssq state (every slot), scale and nrm2 where scaled squares
+    are subnormal at deep folds: the reference's one-call block schedule
+    (dmdssq.ccog:60-101), host and device operands."""
+    import torch
+
+    import reproblas_amd as rb
+
+    x = tc.ssq_subnormal_input(dt, seed)
+    d = dt == np.float64
+    O = oracle() if d else oracle_s()
+    eq = tc.same if d else tc.same32
+    u = np.uint64 if d else np.uint32
+    Ye = np.zeros(2 * fold, dtype=dt)
+    se = O.dbdssq(fold, x, 0.0, Ye) if d else O.sbsssq(fold, x, 0.0, Ye)
+    ne = O.rdnrm2(fold, x) if d else O.rsnrm2(fold, x)
+    for dev in (None, gpu):
+        tx = x if dev is None else torch.from_numpy(x).to(dev)
+        Yg = np.zeros(2 * fold, dtype=dt)
+        sg = rb.dbdssq(Yg, tx, 0.0, fold) if d else rb.sbsssq(Yg, tx, 0.0, fold)
+        assert eq(sg, se), (seed, fold, dev)
+        assert np.array_equal(Yg.view(u), Ye.view(u)), (seed, fold, dev, np.flatnonzero(Yg.view(u) != Ye.view(u)))
+        assert eq(rb.rdnrm2(tx, fold) if d else rb.rsnrm2(tx, fold), ne), (seed, fold, dev)

@@ -317,3 +317,71 @@ def test_amax_nan_semantics(name, x, y):
     if Rs is not None:
         rs = Rs.fn["samaxm"](n, vp(xs), 1, vp(ys), 1) if ys is not None else Rs.fn["samax"](n, vp(xs), 1)
         assert same32(rs, gs), (name, rs, gs)
+
+
+# ------------------------------------- nrm2: subnormal squares at deep folds --
+def ssq_subnormal_input(dt, seed):
+    """Large values in late blocks and tiny ones everywhere whose squares at
+    the final scale are subnormal (dmdssq.ccog:60-101 with N_block 2048, or
+    smsssq's 512 for float): the reference's binned state then depends on
+    its block schedule in the last slot of the bin table."""
+    g = np.random.default_rng(7100 + seed)
+    O = oracle() if dt == np.float64 else oracle_s()
+    nb = 2048 if dt == np.float64 else 512
+    if dt == np.float64:
+        big = [(1, float(g.integers(-300, 0))), (3, float(g.integers(0, 300))), (5, float(g.integers(300, 1000)))]
+        lo, hi = -537.0, -511.0
+    else:
+        big = [(1, float(g.integers(-40, 0))), (3, float(g.integers(0, 40))), (5, float(g.integers(40, 120)))]
+        lo, hi = -75.0, -63.0
+    n = 6 * nb + int(g.integers(0, nb))
+    M = max(1.5 * 2.0 ** e for _, e in big)
+    S = float(O.dscale(M)) if dt == np.float64 else float(O.fn["sscale"](np.float32(M)))
+    x = (S * g.uniform(1, 2, n) * np.exp2(g.uniform(lo, hi, n))) * g.choice([-1.0, 1.0], n)
+    for b, e in big:
+        x[b * nb + int(g.integers(0, nb))] = 1.5 * 2.0 ** e
+    return x.astype(dt)
+
+
+SSQ_SUB_CASES = [(np.float64, seed, fold) for seed in range(4) for fold in (27, 30, 40, 52)] + \
+                [(np.float32, seed, fold) for seed in range(4) for fold in (8, 12, 16, 21)]
+
+
+@pytest.mark.parametrize("dt,seed,fold", SSQ_SUB_CASES,
+                         ids=[f"{'d' if c[0] == np.float64 else 's'}{c[1]}_f{c[2]}" for c in SSQ_SUB_CASES])
+def test_ssq_subnormal_squares_match_reference(dt, seed, fold):
+    x = ssq_subnormal_input(dt, seed)
+    d = dt == np.float64
+    O = oracle() if d else oracle_s()
+    Y = np.zeros(2 * fold, dtype=dt)
+    s = O.dbdssq(fold, x, 0.0, Y) if d else O.sbsssq(fold, x, 0.0, Y)
+    eq = same if d else same32
+    R = reference("sisd") if d else reference_s("sisd")
+    if R is not None:
+        Yr = np.zeros(2 * fold, dtype=dt)
+        sr = R.dbdssq(fold, x, 0.0, Yr) if d else R.sbsssq(fold, x, 0.0, Yr)
+        assert eq(sr, s)
+        u = np.uint64 if d else np.uint32
+        assert np.array_equal(Yr.view(u), Y.view(u)), (seed, fold)
+        assert eq(R.rdnrm2(fold, x) if d else R.rsnrm2(fold, x), O.rdnrm2(fold, x) if d else O.rsnrm2(fold, x))
+
+
+def test_ssq_subnormal_inputs_reach_the_schedule_dependent_slot():
+    """The family above is not vacuous: split into two calls merged by
+    dbdbaddsq, the reference's state differs inside the bin table."""
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    R = reference("sisd") or oracle()
+    O = oracle()
+    differs = 0
+    for seed in range(4):
+        for fold in (30, 40):
+            x = ssq_subnormal_input(np.float64, seed)
+            Y = np.zeros(2 * fold)
+            R.dbdssq(fold, x, 0.0, Y)
+            A, B = np.zeros(2 * fold), np.zeros(2 * fold)
+            sa = R.dbdssq(fold, np.ascontiguousarray(x[:3000]), 0.0, A)
+            sb = R.dbdssq(fold, np.ascontiguousarray(x[3000:]), 0.0, B)
+            R.fn["dbdbaddsq"](fold, sa, vp(A), sb, vp(B))
+            keep = table_slots(fold, Y, O)
+            differs += not np.array_equal(Y[keep].view(np.uint64), B[keep].view(np.uint64))
+    assert differs > 0
