@@ -49,6 +49,7 @@ struct GridAcc {
   int pad;
   int64_t fi;                        // max of INT64_MAX - first infinite term (poisoned values)
 };
+constexpr size_t kReplayFlagOffset = 128;  // nrm2: int, set when the replay kernel must run
 constexpr size_t kGridAccOffset = 256;
 constexpr size_t kSyncBytes = kGridAccOffset + sizeof(GridAcc);  // ticket block of one launch
 constexpr int kGridAccWords = (int)(sizeof(GridAcc) / sizeof(int64_t));
@@ -63,6 +64,7 @@ struct Blas1Args {
   int64_t incy;
   F* partials;    // nrm2: gridDim.x records of (2*fold+1) elements
   F* mins;        // nrm2: gridDim.x smallest nonzero |x| of each workgroup
+  int replay;     // nrm2: ssq_replay_kernel follows this launch (see replay_ssq)
   unsigned* ticket;  // zeroed block of kSyncBytes: the ticket, then a GridAcc
   const F* state_in;
   F scale_in;
@@ -148,7 +150,7 @@ constexpr int ref_block() {
 // nrm2: every term is squared at scale0, which must be at least dscale of
 // every |x| in the range (no wave changes scale).
 template <int CAP, int T, int U, int MODE, class F>
-__device__ void wg_range_sum(const Blas1Args<F>& a, int fold, int64_t lo, int64_t hi, int wave, int lane,
+__device__ __forceinline__ void wg_range_sum(const Blas1Args<F>& a, int fold, int64_t lo, int64_t hi, int wave, int lane,
                              F (*lds)[2 * CAP + 1], State<CAP, F>& out, F scale0 = F(0)) {
   const Span<F> sp = make_span<U>(a.x + lo * a.incx, a.incx, T == kDot ? a.y + lo * a.incy : nullptr, a.incy,
                                   hi - lo);
@@ -180,7 +182,7 @@ __device__ void wg_range_sum(const Blas1Args<F>& a, int fold, int64_t lo, int64_
 // (which is then the result: later blocks add only same-sign infinities or
 // finite terms to it).
 template <int CAP, int T, int U, int MODE, class F>
-__device__ F replay_infinity_path(const Blas1Args<F>& a, int fold, int64_t pstar, int wave, int lane,
+__device__ __forceinline__ F replay_infinity_path(const Blas1Args<F>& a, int fold, int64_t pstar, int wave, int lane,
                                   F (*lds)[2 * CAP + 1], F* seq) {
   constexpr int64_t NB = ref_block<T, F>();
   constexpr int64_t kPer = Fmt<F>::endurance / NB;  // blocks per renorm
@@ -250,11 +252,13 @@ RB_UNROLL_CAP
 // exactly at its scale, with the accumulator rescaled at every epoch start.
 // The result equals the parallel one unless a scaled square is subnormal and
 // the value's lowest bin is fine enough to hold part of it (bin 51 for
-// double, 19-20 for float); the last workgroup then replays this schedule. Called by every thread of the last
-// workgroup; thread 0 ends with the value (y) and its scale (y.scl).
+// double, 19-20 for float); the last workgroup then replays this schedule.
+// Called by every thread of one workgroup (ssq_replay_kernel, its own launch
+// so the streaming kernel keeps its registers); thread 0 ends with the value
+// (y) and its scale (y.scl).
 template <int CAP, int U, int MODE, class F>
-__device__ void replay_ssq(const Blas1Args<F>& a, int fold, int wave, int lane, F (*lds)[2 * CAP + 1],
-                           State<CAP, F>& y) {
+__device__ __forceinline__ void replay_ssq(const Blas1Args<F>& a, int fold, int wave, int lane, F (*lds)[2 * CAP + 1],
+                                           State<CAP, F>& y) {
   constexpr int64_t NB = ref_block<kSSq, F>();
   constexpr int kMaxEpochs = 96;  // scales are distinct powers 2^(WIDTH*k+c): far fewer
   const int64_t nb = (a.n + NB - 1) / NB;
@@ -331,6 +335,27 @@ RB_UNROLL_CAP
   }
 }
 
+// The outputs of a reduction from its final (renormalised) value y.
+template <int T, int CAP, class F>
+__device__ __forceinline__ void write_outputs(const Blas1Args<F>& a, int fold, const State<CAP, F>& y) {
+  F scale = y.scl;
+  if (T == kSSq && nan_or_inf(y.p[0])) scale = F(1);  // dscale(1.0), dmdssq.ccog tail
+  if (a.state_out) {
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        a.state_out[j] = y.p[j];
+        a.state_out[fold + j] = y.c[j];
+      }
+  }
+  if (a.scale_out) *a.scale_out = scale;
+  if (a.result) {
+    const F v = convert(fold, y.p, y.c);
+    // nrm2: scl * sqrt(conv) in double, as rdnrm2.c:31 and rsnrm2.c compute it
+    *a.result = (T == kSSq) ? (F)((double)scale * sqrt((double)v)) : v;
+  }
+}
+
 template <int CAP, bool FIXED, int T, int U, int MODE, class F>
 __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
   constexpr bool kInfPath = (T == kSum || T == kDot);
@@ -346,7 +371,7 @@ __global__ __launch_bounds__(kThreads) void blas1_kernel(Blas1Args<F> a) {
   const F scale0 = (T == kSSq && a.state_in) ? (a.scale_in_dev ? *a.scale_in_dev : a.scale_in) : F(0);
   F wmin = (F)__builtin_inf();  // nrm2: smallest nonzero |x| of the wave
   wave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s, kInfPath ? &pi : nullptr, scale0,
-                                   T == kSSq ? &wmin : nullptr);
+                                   T == kSSq ? &wmin : nullptr, T == kSSq && a.replay);
 
   // Workgroup merge through LDS, then one partial per workgroup.
   __shared__ F lds[kWaves][2 * CAP + 1];
@@ -542,6 +567,7 @@ RB_UNROLL_CAP
     for (int i = threadIdx.x; i < kGridAccWords; i += kThreads) reinterpret_cast<int64_t*>(g)[i] = 0;
   }
   State<CAP, F> y;
+  bool ssq_replay = false;  // thread 0, nrm2: leave the outputs to ssq_replay_kernel
   if (threadIdx.x == 0) {
     // caller's value, renormalised first (the reference ends with a renorm)
     if (a.state_in) {
@@ -590,38 +616,41 @@ RB_UNROLL_CAP
     // bins 19-20 for float) and a term whose square at the final scale is
     // subnormal (|x| / scale < 2^-(MAX_EXP/2 - 1); the reference's earlier
     // block scales are no larger, so its squares are no smaller).
-    __shared__ int ssq_replay;
     if (threadIdx.x == 0) {
       const bool fin = !nan_or_inf(y.p[0]) && y.p[0] != F(0);
       const F tiny = fp_from_bits<F>((typename Fmt<F>::U)(Lay<F>::ieee_bias - (Fmt<F>::max_exp / 2 - 1))
                                      << Lay<F>::mbits);
       const bool fine = fin && bin_exp<F>(dmindex(y.p[0]) + fold - 1) < Lay<F>::mbits + 2;
-      ssq_replay = a.n > 0 && fine && y.scl > F(0) && gmin / y.scl < tiny;
-    }
-    __syncthreads();
-    if (ssq_replay) {
-      replay_ssq<CAP, U, MODE>(a, fold, wave, lane, lds, y);
-      if (threadIdx.x == 0) renorm(fold, y.p, y.c);
+      ssq_replay = a.replay && a.n > 0 && fine && y.scl > F(0) && gmin / y.scl < tiny;
     }
   }
   if (threadIdx.x == 0) {
-    F scale = y.scl;
-    if (T == kSSq && nan_or_inf(y.p[0])) scale = F(1);  // dscale(1.0), dmdssq.ccog tail
-    if (a.state_out) {
-RB_UNROLL_CAP
-      for (int j = 0; j < CAP; ++j)
-        if (j < fold) {
-          a.state_out[j] = y.p[j];
-          a.state_out[fold + j] = y.c[j];
-        }
-    }
-    if (a.scale_out) *a.scale_out = scale;
-    if (a.result) {
-      const F v = convert(fold, y.p, y.c);
-      // nrm2: scl * sqrt(conv) in double, as rdnrm2.c:31 and rsnrm2.c compute it
-      *a.result = (T == kSSq) ? (F)((double)scale * sqrt((double)v)) : v;
-    }
+    // nrm2 in the replay region: ssq_replay_kernel (next on the stream)
+    // writes the outputs from the caller's untouched value
+    if (!(T == kSSq && ssq_replay)) write_outputs<T>(a, fold, y);
+    if (T == kSSq && ssq_replay) *reinterpret_cast<int*>(reinterpret_cast<char*>(a.ticket) + kReplayFlagOffset) = 1;
     *a.ticket = 0u;  // ready for the next launch on this stream
+  }
+}
+
+// One workgroup after an nrm2 launch whose replay switch is on: replays the
+// reference's one-call schedule when that launch flagged the region
+// (replay_ssq), writes the outputs and clears the flag; else returns.
+template <int CAP, bool FIXED, int U, int MODE, class F>
+__global__ __launch_bounds__(kThreads) void ssq_replay_kernel(Blas1Args<F> a) {
+  int* flag = reinterpret_cast<int*>(reinterpret_cast<char*>(a.ticket) + kReplayFlagOffset);
+  if (*flag == 0) return;
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  __shared__ F lds[kWaves][2 * CAP + 1];
+  State<CAP, F> y;
+  replay_ssq<CAP, U, MODE>(a, fold, wave, lane, lds, y);
+  __syncthreads();  // every thread has read the flag
+  if (threadIdx.x == 0) {
+    renorm(fold, y.p, y.c);
+    write_outputs<kSSq>(a, fold, y);
+    *flag = 0;
   }
 }
 
@@ -631,6 +660,15 @@ bool launch_mode(int mode, const Blas1Args<F>& a, int blocks, hipStream_t s) {
     case 0: blas1_kernel<CAP, FIXED, T, U, 0, F><<<blocks, kThreads, 0, s>>>(a); break;
     case 1: blas1_kernel<CAP, FIXED, T, U, 1, F><<<blocks, kThreads, 0, s>>>(a); break;
     default: blas1_kernel<CAP, FIXED, T, U, 2, F><<<blocks, kThreads, 0, s>>>(a); break;
+  }
+  if constexpr (T == kSSq) {
+    if (a.replay) {
+      switch (mode) {
+        case 0: ssq_replay_kernel<CAP, FIXED, U, 0, F><<<1, kThreads, 0, s>>>(a); break;
+        case 1: ssq_replay_kernel<CAP, FIXED, U, 1, F><<<1, kThreads, 0, s>>>(a); break;
+        default: ssq_replay_kernel<CAP, FIXED, U, 2, F><<<1, kThreads, 0, s>>>(a); break;
+      }
+    }
   }
   return RB_HIP(hipGetLastError());
 }
@@ -852,6 +890,12 @@ bool blas1_impl(int fold, Term term, int64_t n, const F* x, int64_t incx, const 
   }
   a.partials = (F*)ws;
   a.mins = ws ? (F*)(ws + (term == kSSq ? rec_bytes : 0)) : nullptr;
+  // nrm2: the region where the reference's block schedule matters (see
+  // replay_ssq) needs the value's lowest bin among the finest ones. Without a
+  // caller's value the scaled squares' top index is 23..27 (double) or 7..11
+  // (float), so only folds >= 25 (double) or >= 9 (float) reach them; a
+  // caller's value of larger scale can put the index anywhere.
+  a.replay = term == kSSq && a.n > 0 && (state_in != nullptr || fold >= (sizeof(F) == 8 ? 25 : 9));
   if (!a.partials || !a.ticket) return false;
   switch (term) {
     case kSum: return blas1_launch_sum(mode, a, (int)blocks, s);

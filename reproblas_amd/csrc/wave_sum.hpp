@@ -457,7 +457,8 @@ struct WaveRun {
   F lim;    // admit_limit(idx)
   F scl;    // nrm2: current scale of the wave
   F inv;    // nrm2: 1 / scl
-  F mn;     // nrm2: smallest nonzero |x| the lane has seen (+inf: none)
+  F mn;     // nrm2: smallest nonzero |x| the lane has seen (+inf: none), when tracked
+  bool track;
   int dep;  // deposits since the last renormalisation
 };
 
@@ -494,7 +495,7 @@ __device__ __forceinline__ void deposit_tile(int fold, const F (&cx)[U], const F
   } else {
     need = (m >= w.lim);
   }
-  if (K == kSSq) {
+  if (K == kSSq && w.track) {
     F mk = (F)__builtin_inf();
 #pragma unroll
     for (int k = 0; k < U; ++k) {
@@ -574,13 +575,15 @@ RB_UNROLL_CAP
 // reference squares every term at max(scaleY, dscale(amax so far))
 // (dmdssq.ccog:88-99), so no term is ever squared at a smaller scale.
 template <int CAP, class F>
-__device__ __forceinline__ void wave_start(int fold, WaveRun<F>& w, State<CAP, F>& s, F scale0 = F(0)) {
+__device__ __forceinline__ void wave_start(int fold, WaveRun<F>& w, State<CAP, F>& s, F scale0 = F(0),
+                                           bool track_min = false) {
   w.idx = Fmt<F>::max_index;
   set_bins(fold, w.idx, s.p, s.c);
   w.lim = admit_limit<F>(w.idx);
   w.scl = scale0;
   w.inv = scale0 > F(0) ? F(1) / scale0 : F(0);  // exact: scales are powers of two
   w.mn = (F)__builtin_inf();
+  w.track = track_min;
   w.dep = 0;
 }
 
@@ -624,13 +627,14 @@ __device__ __forceinline__ void wave_finish(const Span<F>& a, int fold, int64_t 
 // One wave reduces the tiles t0, t0 + tstep, ... of a span into `s` (every
 // lane ends with the wave's value; a poisoned value is replaced by the class
 // of the terms). `info` (or null) receives what the infinity-path replay needs;
-// `min_nz` (or null; nrm2) the smallest nonzero |x| of the wave.
+// `min_nz` (or null; nrm2) the smallest nonzero |x| of the wave when
+// track_min is set (+inf otherwise).
 template <int CAP, int K, int U, int MODE, class F>
 __device__ __forceinline__ void wave_accumulate(const Span<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
                                                 State<CAP, F>& s, PoisonInfo* info = nullptr, F scale0 = F(0),
-                                                F* min_nz = nullptr) {
+                                                F* min_nz = nullptr, bool track_min = false) {
   WaveRun<F> w;
-  wave_start(fold, w, s, scale0);
+  wave_start(fold, w, s, scale0, min_nz != nullptr && track_min);
 
   // Two tile buffers in turn: the loads of the next tile are in flight while
   // the current one is deposited, and no buffer is ever copied.
