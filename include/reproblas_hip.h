@@ -159,6 +159,28 @@ int reproblas_hip_gemm_get_tiling(void);
  * pointers, then launched any number of times; every result is bit-identical
  * to the corresponding single call (reproBLAS_rddot etc. / reproBLAS_rdgemm).
  * Calls that allocate: create and add_* (plan tables); launches do not. */
+/* Order-free exchange of binned values between processes: no counterpart
+ * function in the reference, whose multi-process reduction is an MPI
+ * operator merging pairs in the order MPI chooses (src/binnedMPI/
+ * DBDBADD.ccog, binnedMPI_DBDBADD). A value is encoded as a "wire" of
+ * REPROBLAS_HIP_DB_WIRE (float: _SB_WIRE) int64 words -- the offsets of its
+ * primaries from their bins (in units of the bin's ulp) and its carries, each
+ * at its absolute bin index (index + slot), then counts of NaN, +inf and -inf
+ * classes -- and its index (REPROBLAS_HIP_EMPTY_INDEX for an empty value).
+ * Summing the wires word by word and taking the minimum of the indexes over
+ * any set of values (an all-reduce SUM and MIN), then decoding, gives the
+ * bits that merging them with binned_dbdbadd gives, whatever the order or
+ * grouping: integer addition is associative. For a non-finite result only
+ * the top primary is specified (as for binned_dbdbadd, whose other slots
+ * then depend on the merge order). Host memory; no GPU needed. */
+#define REPROBLAS_HIP_DB_WIRE (2 * 104 + 3)
+#define REPROBLAS_HIP_SB_WIRE (2 * 42 + 3)
+#define REPROBLAS_HIP_EMPTY_INDEX 1048576
+int reproblas_hip_dbencode(int fold, const double_binned *X, int64_t *wire, int64_t *index);
+int reproblas_hip_dbdecode(int fold, const int64_t *wire, int64_t index, double_binned *Y);
+int reproblas_hip_sbencode(int fold, const float_binned *X, int64_t *wire, int64_t *index);
+int reproblas_hip_sbdecode(int fold, const int64_t *wire, int64_t index, float_binned *Y);
+
 typedef struct reproblas_hip_batch reproblas_hip_batch;
 #define REPROBLAS_HIP_OP_SUM 0
 #define REPROBLAS_HIP_OP_ASUM 1

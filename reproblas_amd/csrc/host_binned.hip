@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include "../../include/binned.h"
+#include "../../include/reproblas_hip.h"
 #include "binned_core.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -130,6 +131,104 @@ void binned_dmdmadd(const int fold, const double* priX, const int incpriX, const
 }
 void binned_dbdbadd(const int fold, const double_binned* X, double_binned* Y) {
   binned_dmdmadd(fold, X, 1, X + fold, 1, Y, 1, Y + fold, 1);
+}
+
+}  // extern "C"
+
+// ---- order-free exchange (integer form of binned values) ----
+// Between renormalisations a primary stays in its bin's binade, so its offset
+// from the bin in units of the bin's ulp is a difference of bit patterns;
+// renorm keeps the low mbits-2 bits of an offset in the primary and moves the
+// floor of the rest into the carry. The renormalised sum of values is
+// therefore a function of the per-bin sums of offsets and carries alone (at
+// the lowest index; slots of higher-index values below that window are the
+// ones a raise drops), the same as dmdmadd's pairwise adds give.
+namespace {
+template <class F>
+constexpr int wire_bins() { return Fmt<F>::max_index + Fmt<F>::max_fold + 1; }
+static_assert(wire_bins<double>() == 104 && wire_bins<float>() == 42, "wire sizes of reproblas_hip.h");
+
+template <class F>
+int encode(int fold, const F* X, int64_t* wire, int64_t* index) {
+  clear_error();
+  if (fold < 2 || fold > Fmt<F>::max_fold) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, %d]", fold, Fmt<F>::max_fold);
+    return last_error();
+  }
+  if (!X || !wire || !index) {
+    set_error(kBadArgument, "binned encode: null pointer");
+    return last_error();
+  }
+  constexpr int NB = wire_bins<F>();
+  for (int i = 0; i < 2 * NB + 3; ++i) wire[i] = 0;
+  const F p0 = X[0];
+  if (nan_or_inf(p0)) {
+    wire[2 * NB + (p0 != p0 ? 0 : (p0 > F(0) ? 1 : 2))] = 1;
+    *index = REPROBLAS_HIP_EMPTY_INDEX;
+    return kOk;
+  }
+  if (p0 == F(0)) {
+    *index = REPROBLAS_HIP_EMPTY_INDEX;
+    return kOk;
+  }
+  const int I = dmindex(p0);
+  for (int j = 0; j < fold; ++j) {
+    wire[I + j] = (int64_t)bits(X[j]) - (int64_t)bits(bin<F>(I + j));
+    wire[NB + I + j] = (int64_t)X[fold + j];
+  }
+  *index = I;
+  return kOk;
+}
+
+template <class F>
+int decode(int fold, const int64_t* wire, int64_t index, F* Y) {
+  clear_error();
+  if (fold < 2 || fold > Fmt<F>::max_fold) {
+    set_error(kBadFold, "fold %d is outside the supported range [2, %d]", fold, Fmt<F>::max_fold);
+    return last_error();
+  }
+  constexpr int NB = wire_bins<F>();
+  if (!wire || !Y || index < 0 || (index > Fmt<F>::max_index && index != REPROBLAS_HIP_EMPTY_INDEX)) {
+    set_error(kBadArgument, "binned decode: null pointer or index %lld outside [0, %d]", (long long)index,
+              Fmt<F>::max_index);
+    return last_error();
+  }
+  using U = typename Fmt<F>::U;
+  constexpr int lo = Lay<F>::mbits - 2;
+  if (index == REPROBLAS_HIP_EMPTY_INDEX) {
+    for (int j = 0; j < 2 * fold; ++j) Y[j] = F(0);
+  } else {
+    const int I = (int)index;
+    for (int j = 0; j < fold; ++j) {
+      const int64_t d = wire[I + j];
+      Y[j] = fp_from_bits<F>((U)(bits(bin<F>(I + j)) + (U)(d & (((int64_t)1 << lo) - 1))));
+      Y[fold + j] = (F)(wire[NB + I + j] + (d >> lo));
+    }
+  }
+  const int64_t nan = wire[2 * NB], pinf = wire[2 * NB + 1], ninf = wire[2 * NB + 2];
+  if (nan || (pinf && ninf)) {
+    Y[0] = (F)NAN;
+  } else if (pinf) {
+    Y[0] = (F)INFINITY;
+  } else if (ninf) {
+    Y[0] = -(F)INFINITY;
+  }
+  return kOk;
+}
+}  // namespace
+
+extern "C" {
+int reproblas_hip_dbencode(int fold, const double_binned* X, int64_t* wire, int64_t* index) {
+  return encode<double>(fold, X, wire, index);
+}
+int reproblas_hip_dbdecode(int fold, const int64_t* wire, int64_t index, double_binned* Y) {
+  return decode<double>(fold, wire, index, Y);
+}
+int reproblas_hip_sbencode(int fold, const float_binned* X, int64_t* wire, int64_t* index) {
+  return encode<float>(fold, X, wire, index);
+}
+int reproblas_hip_sbdecode(int fold, const int64_t* wire, int64_t index, float_binned* Y) {
+  return decode<float>(fold, wire, index, Y);
 }
 
 // reference src/binned/dbdbaddv.c:18-23; device arrays run on the GPU

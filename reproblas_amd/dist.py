@@ -1,0 +1,87 @@
+"""Reproducible reduction of binned values across processes (torch.distributed).
+
+The reference reduces binned values across MPI ranks with a user-defined MPI
+operator that merges pairs in whatever order MPI picks
+(src/binnedMPI/DBDBADD.ccog, binnedMPI_DBDBADD); its bits hold because the
+merge is exact. Here every rank encodes its value as a "wire" of int64 words
+(reproblas_hip_dbencode: the offsets of its primaries from their bins and its
+carries at their absolute bin indexes, plus class counts) and its index; one
+all-reduce SUM of the wires and one all-reduce MIN of the indexes, decoded on
+every rank (reproblas_hip_dbdecode), give every rank the bits binned_dbdbadd
+gives merging the values in any order. Integer addition is associative, so
+the ring or tree order of the collective does not matter.
+
+With the gloo backend the tensors are host tensors; with nccl (RCCL) they
+would be device tensors on the rank's GPU. The wire is 211 words for double
+(1.7 KB), 87 for float, whatever the fold.
+"""
+import ctypes
+
+import numpy as np
+
+from . import lib
+
+DB_WIRE = 2 * 104 + 3
+SB_WIRE = 2 * 42 + 3
+EMPTY_INDEX = 1 << 20
+
+
+def _fns():
+    L = lib()
+    out = {}
+    for name, real in (("d", ctypes.c_double), ("s", ctypes.c_float)):
+        enc = getattr(L, f"reproblas_hip_{name}bencode")
+        enc.restype = ctypes.c_int
+        enc.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        dec = getattr(L, f"reproblas_hip_{name}bdecode")
+        dec.restype = ctypes.c_int
+        dec.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        out[name] = (enc, dec)
+    return out
+
+
+def encode(Y, fold):
+    """(wire, index) of the binned value Y (numpy float64 or float32 array of
+    2*fold elements)."""
+    Y = np.ascontiguousarray(Y)
+    kind = "d" if Y.dtype == np.float64 else "s"
+    if Y.dtype not in (np.float64, np.float32) or Y.size != 2 * fold:
+        raise ValueError("Y must hold 2*fold float64 or float32 elements")
+    wire = np.zeros(DB_WIRE if kind == "d" else SB_WIRE, dtype=np.int64)
+    index = np.zeros(1, dtype=np.int64)
+    enc, _ = _fns()[kind]
+    rc = enc(fold, Y.ctypes.data, wire.ctypes.data, index.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"reproblas_hip_{kind}bencode failed ({rc}): {lib().reproblas_hip_last_error_string()}")
+    return wire, int(index[0])
+
+
+def decode(wire, index, fold, dtype=np.float64):
+    """The binned value (2*fold elements of dtype) of a summed wire and the
+    minimum index."""
+    kind = "d" if dtype == np.float64 else "s"
+    wire = np.ascontiguousarray(wire, dtype=np.int64)
+    if wire.size != (DB_WIRE if kind == "d" else SB_WIRE):
+        raise ValueError("wire has the wrong length")
+    Y = np.zeros(2 * fold, dtype=dtype)
+    _, dec = _fns()[kind]
+    rc = dec(fold, wire.ctypes.data, int(index), Y.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"reproblas_hip_{kind}bdecode failed ({rc}): {lib().reproblas_hip_last_error_string()}")
+    return Y
+
+
+def allreduce_binned(Y, fold, group=None):
+    """Replaces this rank's binned value Y (numpy array, 2*fold float64 or
+    float32) by the merge of every rank's value, bit-identical on every rank
+    and for any rank order. Collective: every rank of `group` calls it."""
+    import torch
+    import torch.distributed as dist
+
+    wire, index = encode(Y, fold)
+    tw = torch.from_numpy(wire)
+    ti = torch.tensor([index], dtype=torch.int64)
+    dist.all_reduce(tw, op=dist.ReduceOp.SUM, group=group)
+    dist.all_reduce(ti, op=dist.ReduceOp.MIN, group=group)
+    Y[...] = decode(tw.numpy(), int(ti.item()), fold, dtype=Y.dtype)
+    return Y
