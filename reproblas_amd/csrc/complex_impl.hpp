@@ -71,13 +71,14 @@ constexpr int kWaves = kThreads / 64;
 // Unit-stride workgroups per CU of the complex reductions and persistent
 // workgroups per CU of the complex row-dot gemv (compile-time overridable for
 // tuning builds). Measured on MI355X (2^27 double complex, fold 3): 1/CU zsum
-// 6.0, dznrm2 5.5, zdotc 6.2 TB/s; 2/CU 6.2, 6.1, 6.5 TB/s; the row-dot gemv
-// is within noise at 2, 4 and 8.
+// 6.0, dznrm2 5.5, zdotc 6.2 TB/s; 2/CU 6.2, 6.1, 6.5 TB/s. Row-dot gemv
+// (zgemv 8192^2 column-major T, fold 3, 110 VGPRs: 4 waves per SIMD resident):
+// 2/CU 248 us, 4/CU 216-231 us, 8/CU (one row per wave) 208-221 us.
 #ifndef RB_ZBLAS1_BLOCKS_PER_CU
 #define RB_ZBLAS1_BLOCKS_PER_CU 2
 #endif
 #ifndef RB_ZROW_BLOCKS_PER_CU
-#define RB_ZROW_BLOCKS_PER_CU 4
+#define RB_ZROW_BLOCKS_PER_CU 8
 #endif
 
 
@@ -109,15 +110,23 @@ __device__ __forceinline__ ZSpan<F> make_zspan(const F* x, int64_t incx, const F
   return s;
 }
 
+// What the caller knows of tile t: nothing (the load checks), that every
+// element is in range (no checks, no branch: the registers a pipelined caller
+// prefetches into are then written by the loads alone, so the loads stay in
+// flight until the deposit that needs them), or that it is the partial last
+// tile.
+constexpr int kTileAny = 0, kTileFull = 1, kTilePartial = 2;
+
 // The U complex elements of tile t for this lane. MODE 0: unit stride,
-// 16-byte aligned: 16-byte non-temporal loads (one double complex or two
-// float complex elements each, lanes on consecutive addresses); MODE 1: unit
+// 16-byte aligned: 16-byte loads (one double complex or two float complex
+// elements each, lanes on consecutive addresses), non-temporal for x and for
+// y unless Y_REUSED (a vector every wave re-reads, kept in L2); MODE 1: unit
 // stride, element loads; MODE 2: any stride. Out-of-range elements read as 0.
-template <int T, int U, int MODE, class F>
+template <int T, int U, int MODE, int KNOWN = kTileAny, bool Y_REUSED = false, class F>
 __device__ __forceinline__ void load_ztile(const ZSpan<F>& a, int64_t t, int lane, F (&xr)[U], F (&xi)[U],
                                            F (&yr)[U], F (&yi)[U]) {
   const int64_t base = t * (64 * U);
-  const bool full = t < a.full_tiles;
+  const bool full = KNOWN == kTileFull || (KNOWN == kTileAny && t < a.full_tiles);
   if (MODE == 0 && full) {
     if constexpr (sizeof(F) == 8) {
       const d2* px = reinterpret_cast<const d2*>(a.x + 2 * base);
@@ -128,7 +137,7 @@ __device__ __forceinline__ void load_ztile(const ZSpan<F>& a, int64_t t, int lan
         xr[k] = v.x;
         xi[k] = v.y;
         if (is_dot(T)) {
-          const d2 w = __builtin_nontemporal_load(py + k * 64 + lane);
+          const d2 w = Y_REUSED ? py[k * 64 + lane] : __builtin_nontemporal_load(py + k * 64 + lane);
           yr[k] = w.x;
           yi[k] = w.y;
         }
@@ -144,7 +153,7 @@ __device__ __forceinline__ void load_ztile(const ZSpan<F>& a, int64_t t, int lan
         xr[2 * k + 1] = v.z;
         xi[2 * k + 1] = v.w;
         if (is_dot(T)) {
-          const f4 w = __builtin_nontemporal_load(py + k * 64 + lane);
+          const f4 w = Y_REUSED ? py[k * 64 + lane] : __builtin_nontemporal_load(py + k * 64 + lane);
           yr[2 * k] = w.x;
           yi[2 * k] = w.y;
           yr[2 * k + 1] = w.z;
@@ -437,9 +446,10 @@ constexpr int z_units() {
 // ------------------------------------------------------ complex row gemv --
 // Rows of op(A) contiguous (row-major N, column-major T/C): output o is the
 // complex dotu (conj: dotc) of row o, A + 2*o*lda, with xs = alpha*x. A
-// persistent grid of waves owns rows o, o + W, ... and walks their tiles as
-// one sequence (the next tile -- at a row's end, the first of the next row --
-// in flight while the current one is deposited), as the real row-dot gemv.
+// persistent grid of waves owns rows o, o + W, ... and walks their full tiles
+// as one sequence (the next tile -- at a row's end, the first of the next row
+// -- in flight while the current one is deposited), as the real row-dot gemv;
+// a row's partial last tile is loaded and deposited on its own at its end.
 // Output o accumulates into the real binned values R[2o] (real part) and
 // R[2o+1] (imaginary part).
 template <class F>
@@ -489,9 +499,8 @@ RB_UNROLL_CAP
 
 // Complex elements per lane per tile of the row-dot gemv (compile-time
 // overridable for tuning builds). 32 bytes of each operand per lane: on
-// MI355X, zgemv 8192^2 column-major T/C at fold 3 runs 4.9-5.0 TB/s with 2
-// double complex per lane against 4.4-4.8 with 4 and 3.8-3.9 with 8 (the
-// smaller tile leaves room for more waves per SIMD).
+// MI355X, zgemv 8192^2 column-major T/C at fold 3 runs 208-221 us with 2
+// double complex per lane, 222-231 us with 4 and 241 us with 1.
 #ifndef RB_ZROW_U_D
 #define RB_ZROW_U_D 2
 #endif
@@ -503,6 +512,21 @@ constexpr int zrow_units() {
   return sizeof(F) == sizeof(double) ? RB_ZROW_U_D : RB_ZROW_U_F;
 }
 
+// The end of row o: its partial last tile (if any, loaded and deposited on
+// its own), the merge into R, and fresh states for the next row.
+template <int CAP, int T, int U, int MODE, class F>
+__device__ __forceinline__ void zend_row(const ZRowArgs<F>& a, int fold, int64_t o, int64_t full, bool part,
+                                         int lane, WaveRun<F> (&w)[2], State<CAP, F> (&s)[2]) {
+  if (part) {
+    F cxr[U], cxi[U], cyr[U], cyi[U];
+    load_ztile<T, U, MODE, kTilePartial, true>(zrow_span<U>(a, o), full, lane, cxr, cxi, cyr, cyi);
+    deposit_ztile<CAP, T, U>(fold, cxr, cxi, cyr, cyi, w, s);
+  }
+  zfinish_row<CAP, T, U, MODE>(a, fold, o, lane, w, s);
+  wave_start(fold, w[0], s[0]);
+  wave_start(fold, w[1], s[1]);
+}
+
 template <int CAP, bool FIXED, bool CONJ, int MODE, class F>
 __global__ __launch_bounds__(kThreads) void zrowdot_kernel(ZRowArgs<F> a) {
   constexpr int T = CONJ ? kZDotC : kZDotU;
@@ -512,43 +536,43 @@ __global__ __launch_bounds__(kThreads) void zrowdot_kernel(ZRowArgs<F> a) {
   const int64_t nw = (int64_t)gridDim.x * kWaves;
   int64_t o = (int64_t)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (o >= a.nout) return;  // whole wave leaves together
-  const int64_t tiles = (a.len + 64 * U - 1) / (64 * U);
+  const int64_t full = a.len / (64 * U);
+  const bool part = full * (64 * U) < a.len;
   State<CAP, F> s[2];
   WaveRun<F> w[2];
   wave_start(fold, w[0], s[0]);
   wave_start(fold, w[1], s[1]);
+  if (full == 0) {  // rows shorter than a tile: one partial tile each
+    for (; o < a.nout; o += nw) zend_row<CAP, T, U, MODE>(a, fold, o, full, part, lane, w, s);
+    return;
+  }
+  // The full tiles of rows o, o + W, ... as one sequence, the next one in
+  // flight while the current one is deposited. The prefetch is unconditional
+  // (past the last row it reloads tile 0 of the current row, unused).
   F axr[U], axi[U], ayr[U], ayi[U], bxr[U], bxi[U], byr[U], byi[U];
   int64_t t = 0;
-  load_ztile<T, U, MODE>(zrow_span<U>(a, o), t, lane, axr, axi, ayr, ayi);
+  load_ztile<T, U, MODE, kTileFull, true>(zrow_span<U>(a, o), t, lane, axr, axi, ayr, ayi);
   while (true) {
     int64_t on = o, tn = t + 1;
-    if (tn == tiles) {
+    if (tn == full) {
       on = o + nw;
       tn = 0;
     }
-    if (on < a.nout) load_ztile<T, U, MODE>(zrow_span<U>(a, on), tn, lane, bxr, bxi, byr, byi);
+    load_ztile<T, U, MODE, kTileFull, true>(zrow_span<U>(a, on < a.nout ? on : o), tn, lane, bxr, bxi, byr, byi);
     deposit_ztile<CAP, T, U>(fold, axr, axi, ayr, ayi, w, s);
-    if (tn == 0) {
-      zfinish_row<CAP, T, U, MODE>(a, fold, o, lane, w, s);
-      wave_start(fold, w[0], s[0]);
-      wave_start(fold, w[1], s[1]);
-    }
+    if (tn == 0) zend_row<CAP, T, U, MODE>(a, fold, o, full, part, lane, w, s);
     if (on >= a.nout) break;
     o = on;
     t = tn;
     on = o;
     tn = t + 1;
-    if (tn == tiles) {
+    if (tn == full) {
       on = o + nw;
       tn = 0;
     }
-    if (on < a.nout) load_ztile<T, U, MODE>(zrow_span<U>(a, on), tn, lane, axr, axi, ayr, ayi);
+    load_ztile<T, U, MODE, kTileFull, true>(zrow_span<U>(a, on < a.nout ? on : o), tn, lane, axr, axi, ayr, ayi);
     deposit_ztile<CAP, T, U>(fold, bxr, bxi, byr, byi, w, s);
-    if (tn == 0) {
-      zfinish_row<CAP, T, U, MODE>(a, fold, o, lane, w, s);
-      wave_start(fold, w[0], s[0]);
-      wave_start(fold, w[1], s[1]);
-    }
+    if (tn == 0) zend_row<CAP, T, U, MODE>(a, fold, o, full, part, lane, w, s);
     if (on >= a.nout) break;
     o = on;
     t = tn;
