@@ -501,6 +501,45 @@ RB_UNROLL_CAP
 // reference src/binned/dmdmaddsq.c:26-41, smsmaddsq.c: (scaleY, Y) += (scaleX, X).
 // A non-finite top primary on either side carries the class through p[0]
 // as the plain add does.
+// reference src/binned/zmdrescale.c: both components of a complex binned
+// value (p0/c0 real part, p1/c1 imaginary part) divided slot by slot by
+// (X/scaleY)^2. Its zero test reads the interleaved primaries at i -- slot
+// i/2 of component i%2 -- and on a zero it applies zmdupdate(fold - i, 0.0)
+// to slots i.. of both components: a component whose slot i is zero gets the
+// empty bins from i on (dmdupdate.c on an empty value); a nonzero one keeps
+// its index (0.0 has the largest).
+template <int CAP, class T>
+RB_HD void zrescale(int fold, T X, T scaleY, T (&p0)[CAP], T (&c0)[CAP], T (&p1)[CAP], T (&c1)[CAP]) {
+  if (X == scaleY || X == T(0) || scaleY == T(0)) return;
+  T r = X / scaleY;
+  r *= r;
+  for (int i = 0; i < fold; ++i) {
+    const T v0 = pick(p0, i) / r, v1 = pick(p1, i) / r;
+RB_UNROLL_CAP
+    for (int t = 0; t < CAP; ++t)
+      if (t == i) {
+        p0[t] = v0;
+        p1[t] = v1;
+      }
+    const T z = (i & 1) ? pick(p1, i >> 1) : pick(p0, i >> 1);
+    if (z == T(0)) {
+RB_UNROLL_CAP
+      for (int t = 0; t < CAP; ++t)
+        if (t >= i && t < fold) {
+          if (v0 == T(0)) {
+            p0[t] = bin<T>(Fmt<T>::max_index + (t - i));
+            c0[t] = T(0);
+          }
+          if (v1 == T(0)) {
+            p1[t] = bin<T>(Fmt<T>::max_index + (t - i));
+            c1[t] = T(0);
+          }
+        }
+      return;
+    }
+  }
+}
+
 template <int CAP, class T>
 RB_HD T add_scaled(int fold, T sx, const T (&px)[CAP], const T (&cx)[CAP], T sy, T (&py)[CAP], T (&cy)[CAP]) {
   if (sx > sy) {

@@ -40,7 +40,9 @@ struct ZArgs {
   const F* y;
   int64_t incy;
   F* partials;  // gridDim.x records of 2*(2*fold+1) elements
-  unsigned* ticket;
+  F* mins;      // nrm2: gridDim.x smallest nonzero |component| of each workgroup
+  int replay;   // nrm2: zssq_replay_kernel follows this launch (see zssq_replay_kernel)
+  unsigned* ticket;  // [0] the ticket, [1] nrm2: the replay flag
   const F* state_in;  // complex binned (sum/dot) or real binned (asum/ssq)
   F scale_in;
   const F* scale_in_dev;
@@ -245,12 +247,14 @@ __device__ __forceinline__ void deposit_ztile(int fold, const F (&xr)[U], const 
 // One wave reduces the tiles t0, t0 + tstep, ... into both channels.
 template <int CAP, int T, int U, int MODE, class F>
 __device__ __forceinline__ void zwave_accumulate(const ZSpan<F>& a, int fold, int64_t t0, int64_t tstep, int lane,
-                                                 State<CAP, F> (&s)[2], F scale0 = F(0)) {
+                                                 State<CAP, F> (&s)[2], F scale0 = F(0), F* min_nz = nullptr,
+                                                 bool track_min = false) {
   WaveRun<F> w[2];
-  // nrm2: both channels square at no less than the caller's scale
-  // (dmzssq.ccog, as dmdssq.ccog:88-99)
-  wave_start(fold, w[0], s[0], scale0);
-  wave_start(fold, w[1], s[1], scale0);
+  // nrm2: both channels square at no less than the caller's scale (the
+  // parallel schedule; zssq_replay_kernel replays the reference's where it
+  // matters)
+  wave_start(fold, w[0], s[0], scale0, min_nz != nullptr && track_min);
+  wave_start(fold, w[1], s[1], scale0, min_nz != nullptr && track_min);
   F axr[U], axi[U], ayr[U], ayi[U], bxr[U], bxi[U], byr[U], byi[U];
   int64_t t = t0;
   if (t < a.tiles) load_ztile<T, U, MODE>(a, t, lane, axr, axi, ayr, ayi);
@@ -269,6 +273,16 @@ __device__ __forceinline__ void zwave_accumulate(const ZSpan<F>& a, int fold, in
   for (int ch = 0; ch < 2; ++ch) {
     wave_butterfly(fold, w[ch], s[ch]);
     if (nan_or_inf(uniform_value(s[ch].p[0]))) s[ch].p[0] = zwave_term_class<T, U, MODE>(a, t0, tstep, lane, ch);
+  }
+  if (min_nz) {
+    // nrm2: smallest nonzero |component| of the wave, every lane
+    F m = w[0].mn < w[1].mn ? w[0].mn : w[1].mn;
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+      const F o = lane_partner(m, st);
+      m = o < m ? o : m;
+    }
+    *min_nz = m;
   }
 }
 
@@ -304,15 +318,24 @@ __global__ __launch_bounds__(kThreads) void zblas1_kernel(ZArgs<F> a) {
   const ZSpan<F> sp = make_zspan<U>(a.x, a.incx, a.y, a.incy, a.n);
   State<CAP, F> s[2];
   const F scale0 = (T == kZSSq && a.state_in) ? (a.scale_in_dev ? *a.scale_in_dev : a.scale_in) : F(0);
-  zwave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s, scale0);
+  F wmin = (F)__builtin_inf();  // nrm2: smallest nonzero |component| of the wave
+  zwave_accumulate<CAP, T, U, MODE>(sp, fold, wave_gid, nwaves, lane, s, scale0, T == kZSSq ? &wmin : nullptr,
+                                    T == kZSSq && a.replay);
 
   __shared__ F lds[kWaves][2][2 * CAP + 1];
+  __shared__ F lds_min[kWaves];
   __shared__ bool last;
   const int rec = 2 * (2 * fold + 1);
   {
     State<CAP, F> acc[2];
+    if (T == kZSSq && lane == 0) lds_min[wave] = wmin;  // read after merge_wg's barrier
     merge_wg<CAP, T>(fold, wave, lane, lds, s, acc);
     if (threadIdx.x == 0) {
+      if (T == kZSSq) {
+        F mn = lds_min[0];
+        for (int w = 1; w < kWaves; ++w) mn = lds_min[w] < mn ? lds_min[w] : mn;
+        a.mins[blockIdx.x] = mn;
+      }
       store_state(a.partials + (int64_t)blockIdx.x * rec, fold, acc[0]);
       store_state(a.partials + (int64_t)blockIdx.x * rec + (2 * fold + 1), fold, acc[1]);
       __threadfence();
@@ -328,7 +351,12 @@ __global__ __launch_bounds__(kThreads) void zblas1_kernel(ZArgs<F> a) {
   __threadfence();
   State<CAP, F> acc[2];
   bool have = false;
+  F mn = (F)__builtin_inf();  // nrm2: smallest nonzero |component| of the grid
   for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
+    if (T == kZSSq) {
+      const F bm = __builtin_nontemporal_load(a.mins + b);
+      mn = bm < mn ? bm : mn;
+    }
 #pragma unroll
     for (int ch = 0; ch < 2; ++ch) {
       State<CAP, F> o;
@@ -356,7 +384,20 @@ RB_UNROLL_CAP
   }
   wave_merge_general<CAP, KM>(fold, acc[0]);
   wave_merge_general<CAP, KM>(fold, acc[1]);
+  F gmin = (F)__builtin_inf();  // nrm2, thread 0
+  if (T == kZSSq) {
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+      const F o = lane_partner(mn, st);
+      mn = o < mn ? o : mn;
+    }
+    if (lane == 0) lds_min[wave] = mn;
+  }
   __syncthreads();
+  if (T == kZSSq && threadIdx.x == 0) {
+    gmin = lds_min[0];
+    for (int w = 1; w < kWaves; ++w) gmin = lds_min[w] < gmin ? lds_min[w] : gmin;
+  }
   State<CAP, F> tot[2];
   merge_wg<CAP, T>(fold, wave, lane, lds, acc, tot);
   if (threadIdx.x != 0) return;
@@ -417,6 +458,24 @@ RB_UNROLL_CAP
     // max(scaleY, dscale(amax))): always the scaled merge
     if (a.n > 0) merge<CAP, KM>(fold, y, v);
     renorm(fold, y.p, y.c);
+    if (T == kZSSq && a.replay && a.n > 0 && y.scl > F(0)) {
+      // The reference's one-call schedule where it can differ: a finite
+      // value and a component whose square at the final scale is subnormal
+      // (its earlier blocks squared it at smaller scales, in normal range).
+      // Unlike the real path, no lowest-bin test: the two channels share
+      // one index and zmdrescale's zero test, so the last slot can move at
+      // coarser bins too (fold 27, lowest bin 50). zssq_replay_kernel (next
+      // on the stream) then writes the outputs from the caller's untouched
+      // value.
+      const bool fin = !nan_or_inf(y.p[0]) && y.p[0] != F(0);
+      const F tiny = fp_from_bits<F>((typename Fmt<F>::U)(Lay<F>::ieee_bias - (Fmt<F>::max_exp / 2 - 1))
+                                     << Lay<F>::mbits);
+      if (fin && gmin / y.scl < tiny) {
+        a.ticket[1] = 1u;
+        *a.ticket = 0u;
+        return;
+      }
+    }
     F scale = y.scl;
     if (T == kZSSq && nan_or_inf(y.p[0])) scale = F(1);
     if (a.state_out) {
@@ -434,6 +493,174 @@ RB_UNROLL_CAP
     }
   }
   *a.ticket = 0u;
+}
+
+// The reference's complex sum of squares in one call (dmzssq.ccog:59-135,
+// smcssq.ccog): blocks of 1024 (float: 256) elements in order; before each,
+// a renorm when the deposits would pass the endurance (every second block),
+// then the block's scale dscale(max |component|) -- the accumulator (both
+// channels) rescaled by zmdrescale when it grows -- and an index update of
+// both channels for the block's largest scaled square; then the squares of
+// the real parts into channel 0 and of the imaginary parts into channel 1.
+// The binned state therefore passes through the reference's exact states,
+// unrenormalised ones included, at every rescale. One workgroup: block by
+// block, every lane deposits its share of the block's squares into its own
+// copy of the accumulator's primaries. A slot's slices depend only on the
+// terms and on the slot's ulp, which is the same for the copies and the
+// accumulator (within the endurance every primary stays in its binade; the
+// tiny primaries of slots past the bin table, rescaled below 2^-(MAX_EXP-3),
+// all have the subnormal ulp), so the sum over lanes of each copy's change,
+// counted in ulps (sign times magnitude bits: linear there, across zero
+// too), added to the accumulator gives the reference's primaries. At the
+// end the two channels go into the caller's value by two scaled adds
+// (dmdmaddsq), real part first. Runs only when the streaming kernel flagged
+// the region (ticket[1]); writes the outputs and clears the flag.
+// A value in ulps of the binades with the ulp of the smallest subnormal
+// (and, by differences, of any one binade): sign times magnitude bits.
+template <class F>
+__device__ __forceinline__ int64_t ulp_count(F v) {
+  const int64_t m = (int64_t)(bits(v) & ~((typename Fmt<F>::U)1 << (8 * sizeof(F) - 1)));
+  return v < F(0) ? -m : m;
+}
+template <class F>
+__device__ __forceinline__ F from_ulp_count(int64_t k) {
+  return k < 0 ? -fp_from_bits<F>((typename Fmt<F>::U)(-k)) : fp_from_bits<F>((typename Fmt<F>::U)k);
+}
+
+template <int CAP, bool FIXED, class F>
+__global__ __launch_bounds__(kThreads) void zssq_replay_kernel(ZArgs<F> a) {
+  if (a.ticket[1] == 0u) return;
+  constexpr int64_t NB = sizeof(F) == sizeof(double) ? 1024 : 256;  // dmzssq.ccog:36, smcssq.ccog:36
+  const int fold = FIXED ? CAP : a.fold;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  __shared__ F red[kWaves];
+  __shared__ F bc_inv;
+  __shared__ int bc_idx[2];
+  __shared__ F bc_p[2][CAP];  // the accumulator's primaries before the block's deposits
+  __shared__ bool bc_skip;
+  __shared__ unsigned long long dsum[2][CAP];
+  State<CAP, F> ch[2];  // thread 0: the reference's accumulator
+  F scl = F(0);
+  int deposits = 0;
+  if (threadIdx.x == 0) {
+    set_zero(fold, ch[0].p, ch[0].c);
+    set_zero(fold, ch[1].p, ch[1].c);
+  }
+  const int64_t nb = (a.n + NB - 1) / NB;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t e0 = b * NB, e1 = (b + 1) * NB < a.n ? (b + 1) * NB : a.n;
+    F m = F(0);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kThreads) {
+      const int64_t ix = 2 * e * a.incx;
+      m = fmax(m, fmax(fabs(a.x[ix]), fabs(a.x[ix + 1])));
+    }
+    m = wave_max(m);
+    if (lane == 0) red[wave] = m;
+    for (int i = threadIdx.x; i < 2 * CAP; i += kThreads) (&dsum[0][0])[i] = 0ull;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      F amax = red[0];
+      for (int w = 1; w < kWaves; ++w) amax = fmax(amax, red[w]);
+      if (deposits + (int)(e1 - e0) > Fmt<F>::endurance) {
+        renorm(fold, ch[0].p, ch[0].c);
+        renorm(fold, ch[1].p, ch[1].c);
+        deposits = 0;
+      }
+      const F ns = dscale(amax);
+      if (ns > scl) {
+        if (scl > F(0)) zrescale(fold, ns, scl, ch[0].p, ch[0].c, ch[1].p, ch[1].c);
+        scl = ns;
+      }
+      F am = amax / scl;
+      am = am * am;
+      update(fold, am, ch[0].p, ch[0].c);
+      update(fold, am, ch[1].p, ch[1].c);
+      bc_inv = F(1) / scl;  // exact: scales are powers of two
+      bc_skip = amax == F(0) || ch[0].p[0] == F(0) || ch[1].p[0] == F(0);
+      bc_idx[0] = dmindex(ch[0].p[0]);
+      bc_idx[1] = dmindex(ch[1].p[0]);
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) bc_p[c][j] = ch[c].p[j];
+    }
+    __syncthreads();
+    if (!bc_skip) {  // (an all-zero block deposits nothing but zeros)
+      const F inv = bc_inv;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int I = bc_idx[c];
+        F lp[CAP];
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) lp[j] = bc_p[c][j];
+        for (int64_t e = e0 + threadIdx.x; e < e1; e += kThreads) {
+          F v = a.x[2 * e * a.incx + c] * inv;
+          v = v * v;
+          deposit(fold, I == 0, v, lp);
+        }
+        int64_t d[CAP];
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) d[j] = ulp_count(lp[j]) - ulp_count(bc_p[c][j]);
+RB_UNROLL_CAP
+        for (int j = 0; j < CAP; ++j)
+          if (j < fold) {
+#pragma unroll
+            for (int st = 0; st < 6; ++st) d[j] += lane_partner_i64(d[j], st);
+          }
+        if (lane == 0) {
+RB_UNROLL_CAP
+          for (int j = 0; j < CAP; ++j)
+            if (j < fold) atomicAdd(&dsum[c][j], (unsigned long long)d[j]);
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (!bc_skip) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+RB_UNROLL_CAP
+          for (int j = 0; j < CAP; ++j)
+            if (j < fold) ch[c].p[j] = from_ulp_count<F>(ulp_count(ch[c].p[j]) + (int64_t)dsum[c][j]);
+        }
+      }
+      deposits += (int)(e1 - e0);
+    }
+    __syncthreads();  // red, bc_* and dsum are reused by the next block
+  }
+  if (threadIdx.x != 0) return;
+  renorm(fold, ch[0].p, ch[0].c);
+  renorm(fold, ch[1].p, ch[1].c);
+  State<CAP, F> y;
+  if (a.state_in) {
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        y.p[j] = a.state_in[j];
+        y.c[j] = a.state_in[fold + j];
+      }
+  } else {
+    set_zero(fold, y.p, y.c);
+  }
+  y.scl = a.scale_in_dev ? *a.scale_in_dev : a.scale_in;
+  ch[0].scl = ch[1].scl = scl;
+  merge<CAP, kSSq>(fold, y, ch[0]);
+  merge<CAP, kSSq>(fold, y, ch[1]);
+  if (a.state_out) {
+RB_UNROLL_CAP
+    for (int j = 0; j < CAP; ++j)
+      if (j < fold) {
+        a.state_out[j] = y.p[j];
+        a.state_out[fold + j] = y.c[j];
+      }
+  }
+  if (a.scale_out) *a.scale_out = y.scl;
+  if (a.result) *a.result = (F)((double)y.scl * sqrt((double)convert(fold, y.p, y.c)));
+  a.ticket[1] = 0u;
 }
 
 // Complex elements per lane per tile: 128 bytes of x per lane (64 of each
@@ -629,6 +856,12 @@ bool zlaunch_mode(int mode, const ZArgs<F>& a, int blocks, hipStream_t s) {
     zblas1_kernel<CAP, FIXED, T, U, 0, F><<<blocks, kThreads, 0, s>>>(a);
   else
     zblas1_kernel<CAP, FIXED, T, U, 2, F><<<blocks, kThreads, 0, s>>>(a);
+  if constexpr (T == kZSSq) {
+    if (a.replay) {
+      if (!RB_HIP(hipGetLastError())) return false;
+      zssq_replay_kernel<CAP, FIXED, F><<<1, kThreads, 0, s>>>(a);
+    }
+  }
   return RB_HIP(hipGetLastError());
 }
 
@@ -686,8 +919,17 @@ bool zblas1_impl(int fold, ZTerm term, int64_t n, const F* x, int64_t incx, cons
   const int64_t cap = (int64_t)dev->cus * (mode == 2 ? 4 : RB_ZBLAS1_BLOCKS_PER_CU);
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  a.partials = (F*)workspace(s, kSlotPartials, sizeof(F) * (size_t)blocks * 2 * (2 * fold + 1));
-  a.ticket = tickets(s, 1);
+  // nrm2: the workgroups' smallest nonzero |component| after their records
+  const size_t rec_bytes = sizeof(F) * (size_t)blocks * 2 * (2 * fold + 1);
+  a.partials = (F*)workspace(s, kSlotPartials, rec_bytes + (term == kZSSq ? sizeof(F) * (size_t)blocks : 0));
+  a.mins = a.partials ? (F*)((char*)a.partials + rec_bytes) : nullptr;
+  // nrm2: the region where the reference's block schedule matters (see
+  // zssq_replay_kernel): without a caller's value the scaled squares' top
+  // index is 23..27 (double) or 7..11 (float), so only folds >= 25 (double)
+  // or >= 9 (float) reach it; a caller's value of larger scale can put the
+  // index anywhere.
+  a.replay = term == kZSSq && a.n > 0 && (state_in != nullptr || fold >= (sizeof(F) == 8 ? 25 : 9));
+  a.ticket = tickets(s, 2);
   if (!a.partials || !a.ticket) return false;
   switch (term) {
     case kZSum: return zblas1_launch_sum(fold, mode, a, (int)blocks, s);

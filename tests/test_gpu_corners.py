@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import test_oracle_corners as tc
-from cpu_libs import oracle, oracle_s
+from cpu_libs import oracle, oracle_s, oracle_z
 
 pytestmark = pytest.mark.gpu
 
@@ -145,6 +145,8 @@ def test_ssq_subnormal_squares(gpu, dt, seed, fold):
     u = np.uint64 if d else np.uint32
     Ye = np.zeros(2 * fold, dtype=dt)
     se = O.dbdssq(fold, x, 0.0, Ye) if d else O.sbsssq(fold, x, 0.0, Ye)
+    Ye2 = Ye.copy()
+    se2 = O.dbdssq(fold, x, se, Ye2) if d else O.sbsssq(fold, x, se, Ye2)
     ne = O.rdnrm2(fold, x) if d else O.rsnrm2(fold, x)
     for dev in (None, gpu):
         tx = x if dev is None else torch.from_numpy(x).to(dev)
@@ -152,4 +154,42 @@ def test_ssq_subnormal_squares(gpu, dt, seed, fold):
         sg = rb.dbdssq(Yg, tx, 0.0, fold) if d else rb.sbsssq(Yg, tx, 0.0, fold)
         assert eq(sg, se), (seed, fold, dev)
         assert np.array_equal(Yg.view(u), Ye.view(u)), (seed, fold, dev, np.flatnonzero(Yg.view(u) != Ye.view(u)))
+        # the same terms accumulated into the result (a caller's value)
+        sg2 = rb.dbdssq(Yg, tx, sg, fold) if d else rb.sbsssq(Yg, tx, sg, fold)
+        assert eq(sg2, se2), (seed, fold, dev)
+        assert np.array_equal(Yg.view(u), Ye2.view(u)), (seed, fold, dev, np.flatnonzero(Yg.view(u) != Ye2.view(u)))
         assert eq(rb.rdnrm2(tx, fold) if d else rb.rsnrm2(tx, fold), ne), (seed, fold, dev)
+
+
+@pytest.mark.parametrize("prec,seed,blocks,fold", tc.ZSSQ_SUB_CASES, ids=tc.ZSSQ_SUB_IDS)
+def test_zssq_subnormal_squares(gpu, prec, seed, blocks, fold):
+    """dbzssq / sbcssq state (every slot) and scale, and dznrm2 / scnrm2,
+    where scaled squares are subnormal at deep folds: the reference's
+    one-call block schedule (dmzssq.ccog:59-135), host and device operands,
+    then the same terms accumulated into the result (a caller's value)."""
+    import torch
+
+    import reproblas_amd as rb
+
+    x = tc.zssq_subnormal_input(prec, seed, blocks)
+    z = prec == "z"
+    cx = x.view(np.complex128 if z else np.complex64)
+    Z = oracle_z(prec)
+    u = np.uint64 if z else np.uint32
+    Ye = np.zeros(2 * fold, dtype=x.dtype)
+    se = Z.bssq(fold, x, 0.0, Ye)
+    Ye2 = Ye.copy()
+    se2 = Z.bssq(fold, x, se, Ye2)
+    ne = np.array([Z.rnrm2(fold, x)], x.dtype)
+    ssq = rb.dbzssq if z else rb.sbcssq
+    nrm2 = rb.rdznrm2 if z else rb.rscnrm2
+    for dev in (None, gpu):
+        tx = cx if dev is None else torch.from_numpy(cx).to(dev)
+        Yg = np.zeros(2 * fold, dtype=x.dtype)
+        sg = ssq(Yg, tx, 0.0, fold)
+        assert np.array_equal(np.array([sg], x.dtype).view(u), np.array([se], x.dtype).view(u)), (dev, sg, se)
+        assert np.array_equal(Yg.view(u), Ye.view(u)), (dev, np.flatnonzero(Yg.view(u) != Ye.view(u)))
+        sg2 = ssq(Yg, tx, sg, fold)
+        assert np.array_equal(np.array([sg2], x.dtype).view(u), np.array([se2], x.dtype).view(u)), (dev, sg2, se2)
+        assert np.array_equal(Yg.view(u), Ye2.view(u)), (dev, np.flatnonzero(Yg.view(u) != Ye2.view(u)))
+        assert np.array_equal(np.array([nrm2(tx, fold)], x.dtype).view(u), ne.view(u)), dev

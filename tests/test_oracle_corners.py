@@ -6,9 +6,10 @@ its block schedule rather than on the multiset of terms:
   holding an infinite term the reference adds that block's terms one by one
   to its top primary, so a sequential overflow before the infinity turns the
   result into NaN;
-* the scaled sum of squares (dmdssq.ccog:60-101) over values spanning most
-  of the exponent range with the largest values in late blocks, at every
-  fold, and into an empty accumulator that carries the caller's scale.
+* the scaled sum of squares (dmdssq.ccog:60-101, and its complex form
+  dmzssq.ccog:59-135 / smcssq.ccog) over values spanning most of the
+  exponent range with the largest values in late blocks, at every fold, and
+  into an empty accumulator that carries the caller's scale.
 
 Each case is checked against both builds of the compiled reference
 (oracle/_ref) when present, and always against tests/golden/corners.npz,
@@ -21,7 +22,7 @@ import numpy as np
 import pytest
 
 import datagen as dg
-from cpu_libs import oracle, oracle_s, reference, reference_s
+from cpu_libs import oracle, oracle_s, oracle_z, reference, reference_s, reference_z
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "corners.npz")
 
@@ -364,6 +365,77 @@ def test_ssq_subnormal_squares_match_reference(dt, seed, fold):
         u = np.uint64 if d else np.uint32
         assert np.array_equal(Yr.view(u), Y.view(u)), (seed, fold)
         assert eq(R.rdnrm2(fold, x) if d else R.rsnrm2(fold, x), O.rdnrm2(fold, x) if d else O.rsnrm2(fold, x))
+
+
+def zssq_subnormal_input(prec, seed, blocks):
+    """The complex counterpart (dmzssq.ccog:59-135 with N_block 1024, or
+    smcssq's 256 for float), interleaved (re, im): large components in the
+    given blocks, tiny ones everywhere whose squares at the final scale are
+    subnormal. The reference renormalises before every second block only, so
+    a large value in an odd block rescales an unrenormalised accumulator."""
+    g = np.random.default_rng(7300 + seed)
+    d = prec == "z"
+    O = oracle() if d else oracle_s()
+    nb = 1024 if d else 256
+    if d:
+        exps = [float(g.integers(-300, 0)), float(g.integers(0, 300)), float(g.integers(300, 1000))]
+        lo, hi = -537.0, -511.0
+    else:
+        exps = [float(g.integers(-40, 0)), float(g.integers(0, 40)), float(g.integers(40, 120))]
+        lo, hi = -75.0, -63.0
+    n = 6 * nb + int(g.integers(0, nb))
+    M = max(1.5 * 2.0 ** e for e in exps)
+    S = float(O.dscale(M)) if d else float(O.fn["sscale"](np.float32(M)))
+    x = (S * g.uniform(1, 2, 2 * n) * np.exp2(g.uniform(lo, hi, 2 * n))) * g.choice([-1.0, 1.0], 2 * n)
+    for b, e in zip(blocks, exps):
+        x[2 * (b * nb + int(g.integers(0, nb))) + int(g.integers(0, 2))] = 1.5 * 2.0 ** e
+    return x.astype(np.float64 if d else np.float32)
+
+
+ZSSQ_SUB_CASES = [(prec, seed, blocks, fold) for prec, folds in (("z", (27, 40, 52)), ("c", (9, 12, 21)))
+                  for seed in range(3) for blocks in ((1, 2, 5), (1, 3, 5)) for fold in folds]
+ZSSQ_SUB_IDS = [f"{c[0]}{c[1]}_b{''.join(map(str, c[2]))}_f{c[3]}" for c in ZSSQ_SUB_CASES]
+
+
+@pytest.mark.parametrize("prec,seed,blocks,fold", ZSSQ_SUB_CASES, ids=ZSSQ_SUB_IDS)
+def test_zssq_subnormal_squares_match_reference(prec, seed, blocks, fold):
+    x = zssq_subnormal_input(prec, seed, blocks)
+    Z = oracle_z(prec)
+    u = np.uint64 if prec == "z" else np.uint32
+    Y = np.zeros(2 * fold, dtype=x.dtype)
+    s = Z.bssq(fold, x, 0.0, Y)
+    R = reference_z(prec, "sisd")
+    if R is not None:
+        Yr = np.zeros(2 * fold, dtype=x.dtype)
+        sr = R.bssq(fold, x, 0.0, Yr)
+        assert s == sr
+        assert np.array_equal(Yr.view(u), Y.view(u)), (seed, blocks, fold)
+        assert np.array_equal(np.array([R.rnrm2(fold, x)], x.dtype).view(u), np.array([Z.rnrm2(fold, x)], x.dtype).view(u))
+
+
+def test_zssq_subnormal_inputs_reach_the_schedule_dependent_slots():
+    """The complex family is not vacuous: the same terms in two calls merged
+    by the scaled add give another state for most cases."""
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    for prec in ("z", "c"):
+        Z = oracle_z(prec)
+        O = oracle() if prec == "z" else oracle_s()
+        u = np.uint64 if prec == "z" else np.uint32
+        differs = 0
+        for _, seed, blocks, fold in [c for c in ZSSQ_SUB_CASES if c[0] == prec]:
+            x = zssq_subnormal_input(prec, seed, blocks)
+            Y = np.zeros(2 * fold, dtype=x.dtype)
+            Z.bssq(fold, x, 0.0, Y)
+            h = (len(x) // 4) * 2
+            A, B = np.zeros(2 * fold, dtype=x.dtype), np.zeros(2 * fold, dtype=x.dtype)
+            sa = Z.bssq(fold, np.ascontiguousarray(x[:h]), 0.0, A)
+            sb = Z.bssq(fold, np.ascontiguousarray(x[h:]), 0.0, B)
+            if prec == "z":
+                O.fn["dbdbaddsq"](fold, sa, vp(A), sb, vp(B))
+            else:
+                O.fn["sbsbaddsq"](fold, np.float32(sa), vp(A), np.float32(sb), vp(B))
+            differs += not np.array_equal(Y.view(u), B.view(u))
+        assert differs >= 6, (prec, differs)
 
 
 def test_ssq_subnormal_inputs_reach_the_schedule_dependent_slot():
