@@ -256,6 +256,12 @@ def main():
     # the same call, each of 20 launches between its own events
     per_call = stats(time_each(ddot, 20, 0, stream))
     per_call["GB/s"] = round(16.0 * n / (per_call["median_ms"] / 1e3) / 1e9, 1)
+    # the vendor's (non-reproducible) fp64 dot on the same operands: the rate
+    # a plain streaming read reaches on this GPU, for comparison only
+    t_vendor = time_launches(lambda: torch.dot(x, y), args.steps, args.warmup, stream)
+    vendor = {"kernel": "torch.dot (rocBLAS ddot, not reproducible)", "ms_per_call": round(t_vendor * 1e3, 4),
+              "GB/s": round(16.0 * n / t_vendor / 1e9, 1), "throughput_ratio_ours_to_vendor": round(t_vendor / t, 4)}
+    log(f"[bench] vendor ddot {t_vendor * 1e3:.3f} ms/call, {vendor['GB/s']:.1f} GB/s")
 
     # bit-exactness spot check on a sample against the CPU reference
     cpu = cpu_par = None
@@ -293,6 +299,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(value, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(value / HBM_PEAK_GBS, 4), "traffic": traffic},
         "per_call": per_call,
+        "vendor_ddot": vendor,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_par,
         "parity_sample": None if ref_sample is None else (bits(ref_sample) == bits(gpu_sample)),
