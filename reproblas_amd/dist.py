@@ -14,6 +14,12 @@ the ring or tree order of the collective does not matter.
 With the gloo backend the tensors are host tensors; with nccl (RCCL) they
 would be device tensors on the rank's GPU. The wire is 211 words for double
 (1.7 KB), 87 for float, whatever the fold.
+
+A complex binned value (double_complex_binned / float_complex_binned: the
+primaries of the real and imaginary parts interleaved, then the carries
+likewise) is two real binned values, merged component by component by the
+reference (binned_zbzbadd = dmdmadd on each part, src/binned/zmzmadd.c); it
+travels as two wires and two indexes in the same two all-reduces.
 """
 import ctypes
 
@@ -84,4 +90,35 @@ def allreduce_binned(Y, fold, group=None):
     dist.all_reduce(tw, op=dist.ReduceOp.SUM, group=group)
     dist.all_reduce(ti, op=dist.ReduceOp.MIN, group=group)
     Y[...] = decode(tw.numpy(), int(ti.item()), fold, dtype=Y.dtype)
+    return Y
+
+
+def _parts(Y, fold):
+    """The real and imaginary binned parts (views into Y as a real array of
+    4*fold elements) of a complex binned value."""
+    R = Y.view(np.float64 if Y.dtype in (np.complex128, np.float64) else np.float32).reshape(-1)
+    if R.size != 4 * fold:
+        raise ValueError("a complex binned value holds 4*fold reals (2*fold complex)")
+    return R, [np.concatenate([R[c:2 * fold:2], R[2 * fold + c::2]]) for c in (0, 1)]
+
+
+def allreduce_binned_complex(Y, fold, group=None):
+    """Replaces this rank's complex binned value Y (numpy complex128 / complex64
+    array of 2*fold elements, or the real view of 4*fold) by the merge of every
+    rank's value, bit-identical on every rank and for any rank order: each
+    part as allreduce_binned merges a real value. Collective."""
+    import torch
+    import torch.distributed as dist
+
+    R, parts = _parts(Y, fold)
+    enc = [encode(p, fold) for p in parts]
+    tw = torch.from_numpy(np.concatenate([w for w, _ in enc]))
+    ti = torch.tensor([i for _, i in enc], dtype=torch.int64)
+    dist.all_reduce(tw, op=dist.ReduceOp.SUM, group=group)
+    dist.all_reduce(ti, op=dist.ReduceOp.MIN, group=group)
+    W = tw.numpy().reshape(2, -1)
+    for c in (0, 1):
+        v = decode(W[c], int(ti[c].item()), fold, dtype=R.dtype)
+        R[c:2 * fold:2] = v[:fold]
+        R[2 * fold + c::2] = v[fold:]
     return Y

@@ -7,7 +7,9 @@ Pinned against the oracle: decoding the summed wires of any set of binned
 values gives the bits of merging them one by one with binned_dbdbadd
 (src/binned/dbdbadd.c, dmdmadd.c:27-74), in either order, and the
 all-reduce of per-rank partial sums gives every rank the bits of one binned
-sum over the whole vector, whichever rank holds which shard."""
+sum over the whole vector, whichever rank holds which shard. Complex binned
+values travel as their two parts (binned_zbzbadd merges part by part,
+src/binned/zmzmadd.c)."""
 import os
 import socket
 import tempfile
@@ -16,7 +18,7 @@ import numpy as np
 import pytest
 
 import datagen as dg
-from cpu_libs import oracle, oracle_s
+from cpu_libs import oracle, oracle_s, oracle_z
 
 
 def bits(a):
@@ -83,6 +85,36 @@ def test_wire_classes_and_empty():
     assert np.array_equal(bits(dist.decode(*[np.zeros(dist.DB_WIRE, np.int64), dist.EMPTY_INDEX], fold)), bits(e))
 
 
+@pytest.mark.parametrize("prec,fold", [("z", 3), ("z", 8), ("c", 3)])
+def test_complex_parts_sum_equals_sequential_merges(prec, fold):
+    """Summed wires of each part of complex partial sums decode to the bits
+    of merging the complex values one by one with binned_zbzbadd."""
+    from reproblas_amd import dist
+
+    Z = oracle_z(prec)
+    rt = np.float64 if prec == "z" else np.float32
+    g = np.random.default_rng(41 + fold)
+    vals = []
+    for i in range(5):
+        n = int(g.integers(1, 2000))
+        x = (g.uniform(-1, 1, 2 * n) * np.exp2(g.integers(-20, 20))).astype(rt)
+        Y = np.zeros(4 * fold, dtype=rt)
+        if i != 1:
+            Z.bsum(fold, x, Y)
+        vals.append(Y)
+    enc = [[dist.encode(p, fold) for p in dist._parts(v, fold)[1]] for v in vals]
+    got = np.zeros(4 * fold, dtype=rt)
+    for c in (0, 1):
+        v = dist.decode(np.sum([e[c][0] for e in enc], axis=0), min(e[c][1] for e in enc), fold, dtype=rt)
+        got[c:2 * fold:2] = v[:fold]
+        got[2 * fold + c::2] = v[fold:]
+    for order in (vals, vals[::-1]):
+        exp = np.zeros(4 * fold, dtype=rt)
+        for v in order:
+            Z.fn["bbadd"](fold, Z.p(v), Z.p(exp))
+        assert np.array_equal(bits(got), bits(exp)), (prec, fold)
+
+
 def _free_port():
     with socket.socket() as s:
         s.bind(("127.0.0.1", 0))
@@ -126,3 +158,40 @@ def test_allreduce_over_gloo_matches_one_sum(world):
         for r in range(world):
             assert np.array_equal(bits(got[r]), bits(full)), (world, perm, r)
         assert O.ddbconv(fold, got[0]) == O.rdsum(fold, x)
+
+
+def _rank_main_z(rank, world, port, shards, fold, out_dir):
+    import torch.distributed as dist_
+
+    from reproblas_amd import dist
+
+    os.environ["MASTER_ADDR"] = "127.0.0.1"
+    os.environ["MASTER_PORT"] = str(port)
+    dist_.init_process_group("gloo", rank=rank, world_size=world)
+    try:
+        Y = np.zeros(4 * fold)
+        for x in shards[rank]:
+            oracle_z("z").bsum(fold, x, Y)
+        dist.allreduce_binned_complex(Y, fold)
+        np.save(os.path.join(out_dir, f"r{rank}.npy"), Y)
+    finally:
+        dist_.destroy_process_group()
+
+
+def test_complex_allreduce_over_gloo_matches_one_sum():
+    import torch.multiprocessing as mp
+
+    fold, world = 3, 2
+    g = np.random.default_rng(5)
+    x = (g.uniform(-1, 1, 2 * 9000) * np.exp2(g.integers(-40, 40, 2 * 9000))).astype(np.float64)
+    x[2 * 777 + 1] = 3.0e250  # the imaginary part of one shard raises that part's index
+    pieces = [np.ascontiguousarray(p) for p in np.split(x, 6)]  # 1500 complex each
+    Z = oracle_z("z")
+    full = np.zeros(4 * fold)
+    Z.bsum(fold, x, full)
+    shards = [pieces[r::world] for r in range(world)]
+    with tempfile.TemporaryDirectory() as d:
+        mp.spawn(_rank_main_z, args=(world, _free_port(), shards, fold, d), nprocs=world, join=True)
+        got = [np.load(os.path.join(d, f"r{r}.npy")) for r in range(world)]
+    for r in range(world):
+        assert np.array_equal(bits(got[r]), bits(full)), r
