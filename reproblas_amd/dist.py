@@ -122,3 +122,42 @@ def allreduce_binned_complex(Y, fold, group=None):
         R[c:2 * fold:2] = v[:fold]
         R[2 * fold + c::2] = v[fold:]
     return Y
+
+
+def rescale(Y, scale, new_scale, fold):
+    """Y (a scaled binned sum of squares at `scale`) re-expressed at
+    new_scale >= scale: binned_dmdrescale / binned_smsrescale in place."""
+    Y = np.ascontiguousarray(Y) if not Y.flags["C_CONTIGUOUS"] else Y
+    L = lib()
+    if Y.dtype == np.float64:
+        f = L.binned_dmdrescale
+        f.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
+                      ctypes.c_void_p, ctypes.c_int]
+    else:
+        f = L.binned_smsrescale
+        f.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_int,
+                      ctypes.c_void_p, ctypes.c_int]
+    f.restype = None
+    f(fold, new_scale, scale, Y.ctypes.data, 1, Y.ctypes.data + Y.itemsize * fold, 1)
+    return Y
+
+
+def allreduce_binned_ssq(Y, scale, fold, group=None):
+    """Scaled binned sums of squares (the value and scale binnedBLAS_dbdssq /
+    sbsssq return) merged over ranks: the scales all-reduced with MAX, every
+    value rescaled to it here, then the values exchanged as allreduce_binned
+    does. Returns the common scale (binned_dbdbaddsq returns the larger one).
+    Bit-identical to merging the values with binned_dbdbaddsq in any order
+    where those rescales are exact, i.e. unless a rescale makes a primary
+    subnormal (there the reference's own result depends on its merge order).
+    Collective."""
+    import torch
+    import torch.distributed as dist
+
+    ts = torch.tensor([float(scale)], dtype=torch.float64)
+    dist.all_reduce(ts, op=dist.ReduceOp.MAX, group=group)
+    S = Y.dtype.type(ts.item())
+    if S != Y.dtype.type(scale):
+        rescale(Y, Y.dtype.type(scale), S, fold)
+    allreduce_binned(Y, fold, group)
+    return S

@@ -195,3 +195,87 @@ def test_complex_allreduce_over_gloo_matches_one_sum():
         got = [np.load(os.path.join(d, f"r{r}.npy")) for r in range(world)]
     for r in range(world):
         assert np.array_equal(bits(got[r]), bits(full)), r
+
+
+def ssq_partials(dt, fold, seed, k=5):
+    """k scaled sums of squares over shards of different magnitude ranges
+    (normal range: every rescale exact), one empty."""
+    g = np.random.default_rng(seed)
+    O = oracle() if dt == np.float64 else oracle_s()
+    out = []
+    for i in range(k):
+        n = int(g.integers(1, 3000))
+        x = (g.uniform(-1, 1, n) * np.exp2(g.integers(-30, 30) * (2 if dt == np.float64 else 1))).astype(dt)
+        Y = np.zeros(2 * fold, dtype=dt)
+        s = 0.0
+        if i != 3:
+            s = (O.dbdssq if dt == np.float64 else O.sbsssq)(fold, x, 0.0, Y)
+        out.append((Y, dt(s)))
+    return out
+
+
+@pytest.mark.parametrize("dt,fold", [(np.float64, 3), (np.float64, 8), (np.float32, 3)])
+def test_ssq_exchange_equals_sequential_scaled_merges(dt, fold):
+    """Rescaled to the largest scale and summed as wires: the bits and scale
+    of merging with binned_dbdbaddsq / sbsbaddsq one by one, in either order."""
+    import ctypes as ct
+
+    from reproblas_amd import dist
+
+    vp = lambda a: ct.c_void_p(a.ctypes.data)
+    O = oracle() if dt == np.float64 else oracle_s()
+    addsq = O.fn["dbdbaddsq" if dt == np.float64 else "sbsbaddsq"]
+    vals = ssq_partials(dt, fold, 900 + fold)
+    S = max(s for _, s in vals)
+    wires = [dist.encode(dist.rescale(Y.copy(), s, S, fold), fold) for Y, s in vals]
+    got = dist.decode(np.sum([w for w, _ in wires], axis=0), min(i for _, i in wires), fold, dtype=dt)
+    for order in (vals, vals[::-1]):
+        E, se = np.zeros(2 * fold, dtype=dt), dt(0.0)
+        for Y, s in order:
+            se = dt(addsq(fold, s, vp(Y), se, vp(E)))
+        assert se == S
+        assert np.array_equal(bits(got), bits(E)), (fold, dt)
+
+
+def _rank_main_ssq(rank, world, port, shards, fold, out_dir):
+    import torch.distributed as dist_
+
+    from reproblas_amd import dist
+
+    os.environ["MASTER_ADDR"] = "127.0.0.1"
+    os.environ["MASTER_PORT"] = str(port)
+    dist_.init_process_group("gloo", rank=rank, world_size=world)
+    try:
+        Y, s = np.zeros(2 * fold), 0.0
+        for x in shards[rank]:
+            s = oracle().dbdssq(fold, x, s, Y)
+        s = dist.allreduce_binned_ssq(Y, s, fold)
+        np.save(os.path.join(out_dir, f"r{rank}.npy"), np.concatenate([Y, [s]]))
+    finally:
+        dist_.destroy_process_group()
+
+
+def test_ssq_allreduce_over_gloo_matches_sequential_merges():
+    """Each rank's dbdssq over its shards, exchanged: every rank gets the
+    bits and scale of merging the ranks' values with dbdbaddsq."""
+    import ctypes as ct
+    import torch.multiprocessing as mp
+
+    fold, world = 3, 2
+    g = np.random.default_rng(17)
+    pieces = [(g.uniform(-1, 1, 4000) * np.exp2(e)).astype(np.float64) for e in (-60, 5, 80, -10)]
+    shards = [pieces[r::world] for r in range(world)]
+    O = oracle()
+    vp = lambda a: ct.c_void_p(a.ctypes.data)
+    E, se = np.zeros(2 * fold), 0.0
+    for r in range(world):
+        Y, s = np.zeros(2 * fold), 0.0
+        for x in shards[r]:
+            s = O.dbdssq(fold, x, s, Y)
+        se = O.fn["dbdbaddsq"](fold, s, vp(Y), se, vp(E))
+    with tempfile.TemporaryDirectory() as d:
+        mp.spawn(_rank_main_ssq, args=(world, _free_port(), shards, fold, d), nprocs=world, join=True)
+        got = [np.load(os.path.join(d, f"r{r}.npy")) for r in range(world)]
+    for r in range(world):
+        assert got[r][-1] == se, r
+        assert np.array_equal(bits(got[r][:-1]), bits(E)), r
