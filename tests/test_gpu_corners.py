@@ -193,3 +193,34 @@ def test_zssq_subnormal_squares(gpu, prec, seed, blocks, fold):
         assert np.array_equal(np.array([sg2], x.dtype).view(u), np.array([se2], x.dtype).view(u)), (dev, sg2, se2)
         assert np.array_equal(Yg.view(u), Ye2.view(u)), (dev, np.flatnonzero(Yg.view(u) != Ye2.view(u)))
         assert np.array_equal(np.array([nrm2(tx, fold)], x.dtype).view(u), ne.view(u)), dev
+
+
+@pytest.mark.parametrize("seed,fold", [(seed, fold) for seed in range(2) for fold in (3, 12, 20, 27, 40, 52)])
+def test_znrm2_wide_range(gpu, seed, fold):
+    """dznrm2 and the dbzssq state over the same 2^2000 range read as complex
+    (the largest values in late 1024-element blocks, so the reference's
+    scale rises block by block): scale, nrm2 and every slot inside the bin
+    table; past it (folds >= 28 here) the reference's own content depends on
+    its schedule, as for the real sum of squares."""
+    import torch
+
+    import reproblas_amd as rb
+
+    x = tc.nrm2_range_input(seed)
+    x = np.ascontiguousarray(x[: len(x) // 2 * 2])
+    cx = x.view(np.complex128)
+    Z, O = oracle_z("z"), oracle()
+    Ye = np.zeros(2 * fold)
+    se = Z.bssq(fold, x, 0.0, Ye)
+    ne = Z.rnrm2(fold, x)
+    keep = tc.table_slots(fold, Ye, O)
+    if fold <= 27:
+        assert len(keep) == 2 * fold
+    for dev in (None, gpu):
+        tx = cx if dev is None else torch.from_numpy(cx).to(dev)
+        Yg = np.zeros(2 * fold)
+        sg = rb.dbzssq(Yg, tx, 0.0, fold)
+        assert tc.same(sg, se), (seed, fold, dev)
+        assert np.array_equal(Yg[keep].view(np.uint64), Ye[keep].view(np.uint64)), (seed, fold, dev)
+        assert tc.same(O.ddbconv(fold, Yg), O.ddbconv(fold, Ye)), (seed, fold, dev)
+        assert tc.same(rb.rdznrm2(tx, fold), ne), (seed, fold, dev)
