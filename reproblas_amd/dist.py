@@ -125,9 +125,11 @@ def allreduce_binned_complex(Y, fold, group=None):
 
 
 def rescale(Y, scale, new_scale, fold):
-    """Y (a scaled binned sum of squares at `scale`) re-expressed at
-    new_scale >= scale: binned_dmdrescale / binned_smsrescale in place."""
-    Y = np.ascontiguousarray(Y) if not Y.flags["C_CONTIGUOUS"] else Y
+    """Y (a scaled binned sum of squares at `scale`, a contiguous numpy array
+    of 2*fold float64 or float32) re-expressed at new_scale >= scale:
+    binned_dmdrescale / binned_smsrescale in place."""
+    if Y.dtype not in (np.float64, np.float32) or Y.size != 2 * fold or not Y.flags["C_CONTIGUOUS"]:
+        raise ValueError("Y must be a contiguous array of 2*fold float64 or float32 elements")
     L = lib()
     if Y.dtype == np.float64:
         f = L.binned_dmdrescale
