@@ -341,6 +341,38 @@ def test_full_size_dot_nrm2(gpu, fold):
 
 
 @pytest.mark.slow
+def test_largest_int_length(gpu):
+    """N = 2^31 - 1, the largest length the reference's int arguments take
+    (16 GiB per double operand): sum, asum, dot and nrm2 of constant vectors,
+    whose binned sums are exact, so the results have closed forms (nrm2: the
+    reference's scale * sqrt(conv), rdnrm2.c:31); float dot rounds n once
+    (ssmconv sums in double)."""
+    import math
+
+    import torch
+
+    import reproblas_amd as rb
+
+    n = (1 << 31) - 1
+    O = oracle()
+    x = torch.full((n,), 0.5, device=gpu, dtype=torch.float64)
+    assert rb.rdsum(x) == 0.5 * n
+    y = torch.full((n,), 3.0, device=gpu, dtype=torch.float64)
+    assert rb.rddot(x, y) == 1.5 * n
+    del y
+    x.fill_(-1.0)
+    assert rb.rdasum(x) == float(n)
+    sc = O.dscale(1.0)
+    assert same(rb.rdnrm2(x), sc * math.sqrt(float(n) / (sc * sc)))
+    del x
+    torch.cuda.empty_cache()
+    xs = torch.ones(n, device=gpu, dtype=torch.float32)
+    assert rb.rsdot(xs, xs) == np.float32(float(n))
+    del xs
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.slow
 def test_full_size_ill_conditioned_sum(gpu):
     """N = 2^28 ill-conditioned sum: N/4 values x, their N/4 negations and N/2
     terms of 1e-12*U(-1,1) noise, in three random orders. The three sums are

@@ -211,3 +211,24 @@ def test_high_fold_gemv_gemm(gpu, fold):
         rb.lib().reproBLAS_rdgemv(fold, b"C", trans.encode(), M, K, 2.0, vp(A), M, vp(xv), 1, 0.5, vp(yg), 1)
         rb._check()
         assert eq_arrays(yg, ye), trans
+
+
+@pytest.mark.slow
+def test_gemv_beyond_int_elements(gpu):
+    """46341 x 46341 column-major A (2^31 + 88 elements, 17 GB: element
+    offsets past the int range while M, N and lda fit it), A and x all ones:
+    every output of N (col-axpy) and T (row-dot) is exactly 46341, and
+    beta = 0.5 adds 0.5 * y exactly."""
+    import torch
+
+    import reproblas_amd as rb
+
+    m = 46341
+    A = torch.ones((m, m), device=gpu, dtype=torch.float64)
+    x = torch.ones(m, device=gpu, dtype=torch.float64)
+    for trans in ("N", "T"):
+        y = torch.full((m,), 2.0, device=gpu, dtype=torch.float64)
+        rb.rdgemv(trans, 1.0, A, x, 0.5, y)
+        assert torch.equal(y, torch.full((m,), float(m) + 1.0, device=gpu, dtype=torch.float64)), trans
+    del A
+    torch.cuda.empty_cache()
