@@ -232,3 +232,22 @@ def test_gemv_beyond_int_elements(gpu):
         assert torch.equal(y, torch.full((m,), float(m) + 1.0, device=gpu, dtype=torch.float64)), trans
     del A
     torch.cuda.empty_cache()
+
+
+@pytest.mark.slow
+def test_gemm_beyond_int_elements(gpu):
+    """rdgemm with a 46341 x 46341 column-major C (2^31 + 88 elements, 17 GB),
+    K = 8, A and B all ones, beta = 0.5 on C = 2: every output is exactly
+    8 + 1 = 9 (output offsets past the int range)."""
+    import torch
+
+    import reproblas_amd as rb
+
+    m, k = 46341, 8
+    A = torch.ones((k, m), device=gpu, dtype=torch.float64)  # m x k column-major, lda = m
+    B = torch.ones((m, k), device=gpu, dtype=torch.float64)  # k x m column-major, ldb = k
+    C = torch.full((m, m), 2.0, device=gpu, dtype=torch.float64)
+    rb.rdgemm("N", "N", 1.0, A, B, 0.5, C, M=m, N=m, K=k, lda=m, ldb=k, ldc=m)
+    assert bool((C == 9.0).all())
+    del C
+    torch.cuda.empty_cache()
