@@ -190,36 +190,51 @@ RB_UNROLL_CAP
   const F aalpha = fabs(g.alpha);
   const int64_t chunks = (g.K + KC - 1) / KC;
 
+  // Staging: this thread's element r of a chunk sits at (li0 + r*dli, lka0 +
+  // r*dlka) of the A tile and (lj0 + r*dlj, lkb0 + r*dlkb) of the B tile, the
+  // same in every chunk: its offset from the chunk's corner is off + r*step,
+  // with the step the same for every thread. Interior tiles and chunks load
+  // without bounds tests.
+  constexpr int RA = (BM * KC) / kThreads, RB = (BN * KC) / kThreads;
+  const int li0 = AI ? tid % BM : tid / KC, lka0 = AI ? tid / BM : tid % KC;
+  constexpr int dli = AI ? 0 : kThreads / KC, dlka = AI ? kThreads / BM : 0;
+  const int lj0 = BK ? tid / KC : tid % BN, lkb0 = BK ? tid % KC : tid / BN;
+  constexpr int dlj = BK ? kThreads / KC : 0, dlkb = BK ? 0 : kThreads / BN;
+  const int64_t offA = li0 * g.sai + lka0 * g.sak, stepA = dli * g.sai + dlka * g.sak;
+  const int64_t offB = lkb0 * g.sbk + lj0 * g.sbj, stepB = dlkb * g.sbk + dlj * g.sbj;
+  const bool rows_in = i0 + BM <= g.M, cols_in = j0 + BN <= g.N;  // wave-uniform
+
   for (int64_t ch = 0; ch < chunks; ++ch) {
     const int64_t k0 = ch * KC;
     // ---- stage A(i0.., k0..) * alpha and B(k0.., j0..) into LDS
+    const bool k_in = k0 + KC <= g.K;
+    const F* Ac = g.A + i0 * g.sai + k0 * g.sak + offA;
+    const F* Bc = g.B + k0 * g.sbk + j0 * g.sbj + offB;
+    if (rows_in && k_in) {
 #pragma unroll
-    for (int r = 0; r < (BM * KC) / kThreads; ++r) {
-      int li, lk;
-      if (AI) {
-        li = tid % BM;
-        lk = tid / BM + (kThreads / BM) * r;
-      } else {
-        lk = tid % KC;
-        li = tid / KC + (kThreads / KC) * r;
+      for (int r = 0; r < RA; ++r) {
+        F v = Ac[r * stepA];
+        if (g.alpha != F(1)) v = v * g.alpha;
+        As[lka0 + r * dlka][li0 + r * dli] = v;
       }
-      const int64_t gi = i0 + li, gk = k0 + lk;
-      F v = (gi < g.M && gk < g.K) ? g.A[gi * g.sai + gk * g.sak] : F(0);
-      if (g.alpha != F(1)) v = v * g.alpha;
-      As[lk][li] = v;
+    } else {
+#pragma unroll
+      for (int r = 0; r < RA; ++r) {
+        const int li = li0 + r * dli, lk = lka0 + r * dlka;
+        F v = (i0 + li < g.M && k0 + lk < g.K) ? Ac[r * stepA] : F(0);
+        if (g.alpha != F(1)) v = v * g.alpha;
+        As[lk][li] = v;
+      }
     }
+    if (cols_in && k_in) {
 #pragma unroll
-    for (int r = 0; r < (BN * KC) / kThreads; ++r) {
-      int lj, lk;
-      if (BK) {
-        lk = tid % KC;
-        lj = tid / KC + (kThreads / KC) * r;
-      } else {
-        lj = tid % BN;
-        lk = tid / BN + (kThreads / BN) * r;
+      for (int r = 0; r < RB; ++r) Bs[lkb0 + r * dlkb][lj0 + r * dlj] = Bc[r * stepB];
+    } else {
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int lj = lj0 + r * dlj, lk = lkb0 + r * dlkb;
+        Bs[lk][lj] = (j0 + lj < g.N && k0 + lk < g.K) ? Bc[r * stepB] : F(0);
       }
-      const int64_t gj = j0 + lj, gk = k0 + lk;
-      Bs[lk][lj] = (gj < g.N && gk < g.K) ? g.B[gk * g.sbk + gj * g.sbj] : F(0);
     }
     __syncthreads();
 
