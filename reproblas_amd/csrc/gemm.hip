@@ -240,15 +240,25 @@ RB_UNROLL_CAP
 
     // ---- index guard for this chunk
     F ra = F(0), cb = F(0);
+    if (rows_in) {  // all loads in flight together
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      const int64_t gi = i0 + tr * TM + i;
-      if (gi < g.M) ra = fmax(ra, g.rowmax[ch * g.M + gi]);
+      for (int i = 0; i < TM; ++i) ra = fmax(ra, g.rowmax[ch * g.M + i0 + tr * TM + i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int64_t gi = i0 + tr * TM + i;
+        if (gi < g.M) ra = fmax(ra, g.rowmax[ch * g.M + gi]);
+      }
     }
+    if (cols_in) {
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int64_t gj = j0 + tc * TN + j;
-      if (gj < g.N) cb = fmax(cb, g.colmax[ch * g.N + gj]);
+      for (int j = 0; j < TN; ++j) cb = fmax(cb, g.colmax[ch * g.N + j0 + tc * TN + j]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int64_t gj = j0 + tc * TN + j;
+        if (gj < g.N) cb = fmax(cb, g.colmax[ch * g.N + gj]);
+      }
     }
     const F bound = (aalpha * ra) * cb;
     if (!(bound < lmin)) {
