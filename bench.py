@@ -218,8 +218,8 @@ def main():
                     help="time a plain device copy and a plain torch sum of 2^28 doubles (scales the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the multi-GPU reduction is a design, not an implementation)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the library runs the reduction on one GPU; the exchange between processes is reproblas_amd.dist)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -1,9 +1,9 @@
 /*
  * oracle.h -- CPU restatement of ReproBLAS 2.1.0's binned FP64 reduction path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the correctness checker for the
+ * TEST INFRASTRUCTURE ONLY. This library is the correctness oracle for the
  * HIP library in reproblas_amd/. Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load it, and only as the checker or as the
+ * cpu_baseline leg of bench.py may load it, and only as the oracle or as the
  * timed CPU baseline. The product never links, loads or calls it.
  *
  * Every symbol is prefixed orc_ so it can never be confused with (or

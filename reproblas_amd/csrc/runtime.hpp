@@ -16,7 +16,7 @@ enum Status : int {
   kBadArgument = 2,  // negative size, bad leading dimension, null pointer
   kHipError = 3,     // a HIP runtime call failed
   kNoDevice = 4,     // no usable GPU
-  kUnsupported = 5,  // a case this build does not handle
+  kUnsupported = 5,  // a case the library does not handle
 };
 
 void set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));

@@ -1,4 +1,4 @@
-"""ctypes handles on the CPU checker libraries (test infrastructure only).
+"""ctypes handles on the CPU oracle libraries (test infrastructure only).
 
 * ``oracle()``  -- oracle/liboracle.so, the scalar restatement (orc_* symbols);
   ``oracle_s()`` the same library's single-precision calls.
@@ -30,7 +30,7 @@ _vp = ctypes.c_void_p
 
 def _ptr(a):
     if not a.flags["C_CONTIGUOUS"] or a.dtype != np.float64:
-        raise ValueError("CPU checker calls need contiguous float64 arrays")
+        raise ValueError("oracle calls need contiguous float64 arrays")
     return ctypes.c_void_p(a.ctypes.data)
 
 
@@ -166,7 +166,7 @@ _c_float = ctypes.c_float
 
 def _fptr(a):
     if not a.flags["C_CONTIGUOUS"] or a.dtype != np.float32:
-        raise ValueError("CPU checker calls need contiguous float32 arrays")
+        raise ValueError("oracle calls need contiguous float32 arrays")
     return ctypes.c_void_p(a.ctypes.data)
 
 
@@ -318,7 +318,7 @@ class CpuBlasZ:
 
     def p(self, a):
         if not a.flags["C_CONTIGUOUS"] or a.dtype != self.dtype:
-            raise ValueError("complex checker calls need contiguous interleaved arrays of the precision's type")
+            raise ValueError("complex oracle calls need contiguous interleaved arrays of the precision's type")
         return ctypes.c_void_p(a.ctypes.data)
 
     def scalar(self, v):

@@ -1,4 +1,4 @@
-"""Pins the oracle (oracle/oracle.c) before it is trusted as the GPU checker.
+"""Pins the oracle (oracle/oracle.c) before it is trusted as the reference for the GPU results.
 
 1. Bit-equality with the compiled reference (oracle/_ref: the reference's own
    d-path C sources, AVX and scalar builds) over random, ill-conditioned,

@@ -1,5 +1,5 @@
 """Pins the single-precision oracle (oracle/oracle_s.c) before it is trusted
-as the checker of the FP32 kernels.
+as the reference for the FP32 kernels.
 
 1. Bit-equality with the compiled reference (oracle/_ref: the reference's own
    s-path C sources, AVX and scalar builds) over random, ill-conditioned,

@@ -1,5 +1,5 @@
 """Pins the complex oracle (oracle/oracle_z.c via oracle_cplx.inc; double
-complex "z" and float complex "c") before it is trusted as the checker of
+complex "z" and float complex "c") before it is trusted as the reference for
 the complex kernels.
 
 1. Bit-equality with the compiled reference (oracle/_ref: the reference's
